@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Headline benchmark: trajectories/s of the mixed-precision persistent tracer on one
+MI355X, against the plain FP64 GPU tracer, on the global 5-degree vertical-cutoff grid.
+
+Workload (one "step" = one pass over the whole batch, inputs resident in HBM):
+  IGRF-14 at 2020-01-01 00:00 plus T96 (Pdyn 2 nPa, Dst -20 nT, By 0, Bz -2 nT);
+  sites at geodetic latitude -90..90 x longitude 0..355 in 5-degree steps (2664) at
+  20 km (WGS84 -> geocentric), vertical incidence; protons back-traced at 0.01..20 GV
+  in 0.01 GV steps (2000 per site, 5.328 M trajectories); nk1 = 100, tu = 0.01 rad,
+  at most 1e6 steps per trajectory.
+
+Printed: one JSON line (value = mixed-precision trajectories/s over the whole batch),
+with the FP64 GPU figures, the mixed-vs-FP64 cutoff agreement, the roofline of the
+tracer kernel and the reference CPU path timed on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "cor-model_amd"))
+
+import numpy as np  # noqa: E402
+
+import cor_amd as A  # noqa: E402
+
+PARMOD = [2.0, -20.0, 0.0, -2.0]        # Pdyn, Dst, By, Bz (reference PARMOD order)
+EPOCH = (2020, 1, 1, 1, 0, 0, 0)
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+
+
+def build_scans(stride, nk1, rig0=0.01, dl=0.01, rk=20.0):
+    sites = A.grid_sites(5.0, 5.0, 20.0)[::stride]
+    rows = [(r0, lat, lon, lat, lon, rig0, dl, rk, -1.0, nk1) for (r0, lat, lon) in sites]
+    return A.scans_array(rows), sites
+
+
+def flop_model():
+    """Per-accepted-step FP operation counts measured with the op-counting build of the
+    oracle (profiles/flop_model.json, counting rules there)."""
+    f = REPO / "profiles" / "flop_model.json"
+    if f.exists():
+        return json.loads(f.read_text())
+    return None
+
+
+def time_plan(plan, steps, warmup):
+    for _ in range(warmup):
+        plan.run()
+        plan.sync()
+    A.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run()
+    A.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) / max(steps, 1)
+
+
+def cpu_baseline(budget_s, nk1, step_limit):
+    """The reference itself (oracle/_ref/cor, built from its sources) on a bounded
+    sample of the same grid: whole sites, -p (OpenMP over the rigidity scan) on this
+    host's cores, until the time budget is spent."""
+    ref = REPO / "oracle" / "_ref" / "cor"
+    if not ref.exists():
+        return None
+    sites = A.grid_sites(5.0, 5.0, 20.0)
+    rng = np.random.default_rng(12345)
+    order = rng.permutation(len(sites))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    done, n_traj, t_total = 0, 0, 0.0
+    with tempfile.TemporaryDirectory() as td:
+        for k in order:
+            r0, lat, lon = sites[k]
+            inp = Path(td) / "in.txt"
+            inp.write_text(f"0.01 -1 20.00\n{r0!r} {lat!r} {lon!r}\n{lat!r} {lon!r}\n"
+                           f"2020 1 1 1 0 0 0\n{nk1} 0 0 0.01\n-20 2 0 -2\n")
+            t0 = time.perf_counter()
+            subprocess.run([str(ref), "-p", "-i", "igrf14", "-e", "t96", "-s", str(step_limit), str(inp)],
+                           check=True, capture_output=True, timeout=600)
+            t_total += time.perf_counter() - t0
+            done += 1
+            n_traj += 2000
+            if t_total >= budget_s and done >= 2:
+                break
+    return {"value": n_traj / t_total, "unit": "trajectories/s", "cores": threads, "kind": "reference",
+            "sample": f"{done} seeded sites of the 5-degree grid x 2000 rigidities "
+                      f"({n_traj} trajectories) in {t_total:.1f} s, oracle/_ref/cor -p"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--stride", type=int, default=1, help="every stride-th site of the grid")
+    ap.add_argument("--nk1", type=int, default=100)
+    ap.add_argument("--step-limit", type=int, default=1000000)
+    ap.add_argument("--queue-order", type=int, default=1)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the tracer shards by "
+              f"station across GPUs, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    A.init(0)
+    scans, sites = build_scans(args.stride, args.nk1)
+    params = A.recalc(*EPOCH, parmod=PARMOD)
+    models = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
+    mixed = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_MIXED,
+                   queue_order=args.queue_order)
+    n = mixed.n
+    print(f"[bench] {len(scans)} sites, {n} trajectories; warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+    t_mixed = time_plan(mixed, args.steps, args.warmup)
+    st_m = mixed.stats()
+    _, cut_m = mixed.results(traj=False)
+    print(f"[bench] mixed: {t_mixed*1e3:.1f} ms/step, kernel {st_m.kernel_ms:.1f} ms, "
+          f"{st_m.steps} steps, {st_m.attempts} attempts", file=sys.stderr, flush=True)
+
+    fp64 = None
+    if not args.no_fp64:
+        p64 = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_FP64,
+                     queue_order=args.queue_order)
+        t64 = time_plan(p64, 1, 0)
+        st_d = p64.stats()
+        _, cut_d = p64.results(traj=False)
+        p64.close()
+        drc = np.abs(cut_m["rc"] - cut_d["rc"])
+        fp64 = {"value": n / t64, "ms_per_step": t64 * 1e3, "kernel_ms": st_d.kernel_ms,
+                "steps": int(st_d.steps), "attempts": int(st_d.attempts),
+                "speedup_mixed_vs_fp64": (n / t_mixed) / (n / t64),
+                "sites_within_0.01GV": float(np.mean(drc <= 0.01 + 1e-9)),
+                "max_abs_dRc_GV": float(drc.max()),
+                "mean_abs_dRl_GV": float(np.mean(np.abs(cut_m["rl"] - cut_d["rl"]))),
+                "mean_abs_dRu_GV": float(np.mean(np.abs(cut_m["ru"] - cut_d["ru"])))}
+        print(f"[bench] fp64: {t64*1e3:.1f} ms/step, kernel {st_d.kernel_ms:.1f} ms; "
+              f"sites within 0.01 GV: {fp64['sites_within_0.01GV']:.4f}", file=sys.stderr, flush=True)
+
+    roof = None
+    fm = flop_model()
+    if fm is not None:
+        f_step = fm["t96_igrf"]["flops_per_step"]
+        achieved = f_step * st_m.steps / (st_m.kernel_ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
+                "flops_per_step": f_step, "steps_per_s": st_m.steps / (st_m.kernel_ms * 1e-3)}
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_budget, args.nk1, args.step_limit)
+    out = {
+        "metric": "trajectories/s on 1 MI355X, mixed vs FP64 GPU, at |dRc|<=0.01 GV vs FP64",
+        "value": n / t_mixed,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 field+RK increments, f64 state",
+        "data": "synthetic",
+        "config": {"workload": "global 5-degree vertical cutoff grid, IGRF-14 2020.0 + T96 "
+                               "(Pdyn 2, Dst -20, By 0, Bz -2), 0.01-20 GV step 0.01",
+                   "sites": len(scans), "site_stride": args.stride, "trajectories": int(n),
+                   "nk1": args.nk1, "step_limit": args.step_limit, "tu_angle": 0.01,
+                   "queue_order": args.queue_order},
+        "kernel_ms": st_m.kernel_ms,
+        "integration_steps_per_s": st_m.steps / (st_m.kernel_ms * 1e-3),
+        "fp64": fp64,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,290 @@
+"""ctypes binding of libcor_amd.so (include/cor.h, include/cor_gpu.h).
+
+This is the Python face of the C-ABI the reference's C callers link against: the
+tests, the smoke check and the benchmark drive the GPU tracer through it.  It loads
+the in-tree library and fails loudly when it is missing -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcor_amd.so"
+
+NCOEF = 105
+DP = C.POINTER(C.c_double)
+
+COR_OK, COR_E_INVALID, COR_E_NO_DEVICE, COR_E_HIP, COR_E_UNSUPPORTED, COR_E_NOMEM = 0, -1, -2, -3, -4, -5
+PREC_FP64, PREC_MIXED = 0, 1
+INTERNAL_NONE, INTERNAL_IGRF = 0, 1
+EXTERNAL_NONE, EXTERNAL_T96, EXTERNAL_TS05 = 0, 1, 2
+CRUSTAL_NONE, CRUSTAL_SH = 0, 1
+ALLOWED, FORBIDDEN_REENTRY, FORBIDDEN_LIMIT = 0, 1, 2
+M_PROTON = 1.6725e-27
+M_ELECTRON = 9.1093826e-31
+RE_M = 6.3712e06
+
+# every symbol include/cor.h and include/cor_gpu.h declare
+EXPORTED = [
+    "get_model", "calculate_field", "sph_to_car", "car_to_sph", "geo_to_gsm", "gsm_to_geo",
+    "igrf14_recalc", "read_cutoff_infile_header", "write_cutoff_outfile_header",
+    "read_field_infile_header", "write_field_outfile_header", "read_ts05_params",
+    "read_t96_params", "trajectory_simulation_cutoff", "field_simulation",
+    "internal_models", "external_models", "crustal_models",
+    "cor_gpu_init", "cor_gpu_error_string", "cor_gpu_last_hip_error", "cor_trace_plan_create",
+    "cor_trace_plan_run", "cor_trace_plan_sync", "cor_trace_plan_count", "cor_trace_plan_results",
+    "cor_trace_plan_stats", "cor_trace_plan_destroy", "cor_gpu_trace", "cor_gpu_field",
+    "cor_scan_count", "cor_geodetic_to_geocentric", "cor_direction_to_geo", "cor_gpu_synchronize",
+]
+
+
+class ModelsParams(C.Structure):
+    _fields_ = [("PARMOD", C.c_double * 10), ("G", C.c_double * NCOEF), ("H", C.c_double * NCOEF),
+                ("REC", C.c_double * NCOEF), ("A1", C.c_double * 3), ("A2", C.c_double * 3),
+                ("A3", C.c_double * 3)]
+
+
+class RecalcTime(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("iy", "mes", "ide", "id", "ih", "min", "is_")]
+
+
+class ModelSet(C.Structure):
+    _fields_ = [("internal", C.c_int32), ("external", C.c_int32), ("crustal", C.c_int32),
+                ("sh_nm_max", C.c_int32), ("sh_g", DP), ("sh_h", DP)]
+
+
+class Scan(C.Structure):
+    _fields_ = [("r0", C.c_double), ("the0", C.c_double), ("fi0", C.c_double),
+                ("the1", C.c_double), ("fi1", C.c_double), ("rig", C.c_double),
+                ("del_", C.c_double), ("rk", C.c_double), ("zn", C.c_double),
+                ("nk1", C.c_int32), ("pad", C.c_int32)]
+
+
+class TraceOpts(C.Structure):
+    _fields_ = [("step_limit", C.c_uint64), ("tu_angle", C.c_double), ("particle_mass", C.c_double),
+                ("precision", C.c_int32), ("queue_order", C.c_int32)]
+
+
+class Cutoff(C.Structure):
+    _fields_ = [("rl", C.c_double), ("ru", C.c_double), ("rc", C.c_double),
+                ("n_forbidden", C.c_int32), ("n", C.c_int32)]
+
+
+class TraceStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("steps", C.c_uint64),
+                ("attempts", C.c_uint64), ("n_traj", C.c_int64), ("grid_blocks", C.c_int32),
+                ("block_threads", C.c_int32)]
+
+
+TRAJ_DTYPE = np.dtype([("status", "<i4"), ("steps", "<u4"), ("attempts", "<u4"), ("pad", "<u4"),
+                       ("vc", "<f8"), ("r", "<f8"), ("td", "<f8"), ("fei", "<f8"), ("ast", "<f8"),
+                       ("asf", "<f8"), ("time", "<f8"), ("length_m", "<f8")])
+SCAN_DTYPE = np.dtype([("r0", "<f8"), ("the0", "<f8"), ("fi0", "<f8"), ("the1", "<f8"),
+                       ("fi1", "<f8"), ("rig", "<f8"), ("del_", "<f8"), ("rk", "<f8"),
+                       ("zn", "<f8"), ("nk1", "<i4"), ("pad", "<i4")])
+CUT_DTYPE = np.dtype([("rl", "<f8"), ("ru", "<f8"), ("rc", "<f8"), ("n_forbidden", "<i4"),
+                      ("n", "<i4")])
+
+
+class CorError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libcor_amd.so (built in-tree by __graft_entry__.build / make)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise CorError(f"{LIB_PATH} is not built: run `make -C cor-model_amd/csrc`")
+    L = C.CDLL(str(LIB_PATH))
+    L.cor_gpu_init.argtypes = [C.c_int]
+    L.cor_gpu_error_string.restype = C.c_char_p
+    L.cor_gpu_error_string.argtypes = [C.c_int]
+    L.cor_gpu_last_hip_error.restype = C.c_char_p
+    L.igrf14_recalc.argtypes = [C.POINTER(RecalcTime)] + [DP] * 6
+    L.igrf14_recalc.restype = C.c_int
+    L.cor_scan_count.argtypes = [C.POINTER(Scan)]
+    L.cor_scan_count.restype = C.c_int64
+    L.cor_trace_plan_create.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p,
+                                        C.c_int32, C.POINTER(TraceOpts), C.POINTER(C.c_void_p)]
+    L.cor_trace_plan_run.argtypes = [C.c_void_p, C.c_void_p]
+    L.cor_trace_plan_sync.argtypes = [C.c_void_p]
+    L.cor_trace_plan_count.argtypes = [C.c_void_p]
+    L.cor_trace_plan_count.restype = C.c_int64
+    L.cor_trace_plan_results.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.cor_trace_plan_stats.argtypes = [C.c_void_p, C.POINTER(TraceStats)]
+    L.cor_trace_plan_destroy.argtypes = [C.c_void_p]
+    L.cor_trace_plan_destroy.restype = None
+    L.cor_gpu_trace.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p, C.c_int32,
+                                C.POINTER(TraceOpts), C.c_void_p, C.c_void_p]
+    L.cor_gpu_field.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), DP, DP, C.c_int64, C.c_int32]
+    L.cor_geodetic_to_geocentric.argtypes = [C.c_double, C.c_double, DP, DP]
+    L.cor_direction_to_geo.argtypes = [C.c_double] * 4 + [DP, DP]
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != COR_OK:
+        L = lib()
+        msg = L.cor_gpu_error_string(rc).decode()
+        hip = L.cor_gpu_last_hip_error().decode()
+        raise CorError(f"{what}: {msg} ({rc}) {hip}")
+
+
+def init(device=0):
+    check(lib().cor_gpu_init(device), "cor_gpu_init")
+
+
+def synchronize():
+    check(lib().cor_gpu_synchronize(), "cor_gpu_synchronize")
+
+
+def recalc(iy, mes=1, ide=1, id_=1, ih=0, mi=0, is_=0, parmod=None):
+    """igrf14_recalc on the host -> ModelsParams (PARMOD filled from `parmod`,
+    in the reference order Pdyn, Dst, By, Bz, W1..W6)."""
+    p = ModelsParams()
+    t = RecalcTime(iy, mes, ide, id_, ih, mi, is_)
+    rc = lib().igrf14_recalc(C.byref(t), p.G, p.H, p.REC, p.A1, p.A2, p.A3)
+    if rc != 0:
+        raise CorError(f"igrf14_recalc rejected epoch {iy}")
+    if parmod is not None:
+        for k, v in enumerate(parmod):
+            p.PARMOD[k] = float(v)
+    return p
+
+
+def model_set(internal=INTERNAL_IGRF, external=EXTERNAL_NONE, sh=None):
+    m = ModelSet(internal, external, CRUSTAL_NONE, 0, None, None)
+    if sh is not None:
+        g = np.ascontiguousarray(sh[0], dtype=np.float64)
+        h = np.ascontiguousarray(sh[1], dtype=np.float64)
+        m._keep = (g, h)
+        m.crustal = CRUSTAL_SH
+        m.sh_nm_max = g.shape[0]
+        m.sh_g = g.ctypes.data_as(DP)
+        m.sh_h = h.ctypes.data_as(DP)
+    return m
+
+
+def scans_array(rows):
+    """rows: iterable of dicts / tuples (r0, the0, fi0, the1, fi1, rig, del, rk, zn, nk1)."""
+    a = np.zeros(len(rows), dtype=SCAN_DTYPE)
+    for i, r in enumerate(rows):
+        if isinstance(r, dict):
+            for k, v in r.items():
+                a[i][k] = v
+        else:
+            (a[i]["r0"], a[i]["the0"], a[i]["fi0"], a[i]["the1"], a[i]["fi1"], a[i]["rig"],
+             a[i]["del_"], a[i]["rk"], a[i]["zn"], a[i]["nk1"]) = r
+    return a
+
+
+def scan_count(row):
+    s = Scan(*[float(row[k]) for k in ("r0", "the0", "fi0", "the1", "fi1", "rig", "del_", "rk", "zn")],
+             int(row["nk1"]), 0)
+    return lib().cor_scan_count(C.byref(s))
+
+
+class Plan:
+    """cor_trace_plan: epoch + scans uploaded to HBM; run() traces them all."""
+
+    def __init__(self, models, params, scans, step_limit=1000000, tu_angle=0.01, mass=M_PROTON,
+                 precision=PREC_FP64, queue_order=0):
+        self.scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
+        opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order)
+        h = C.c_void_p()
+        self._models = models
+        check(lib().cor_trace_plan_create(C.byref(models), C.byref(params), self.scans.ctypes.data,
+                                          len(self.scans), C.byref(opts), C.byref(h)), "plan_create")
+        self.h = h
+        self.n = lib().cor_trace_plan_count(h)
+
+    def run(self, stream=None):
+        check(lib().cor_trace_plan_run(self.h, C.c_void_p(stream) if stream else None), "plan_run")
+
+    def sync(self):
+        check(lib().cor_trace_plan_sync(self.h), "plan_sync")
+
+    def results(self, traj=True):
+        cut = np.zeros(len(self.scans), dtype=CUT_DTYPE)
+        tr = np.zeros(self.n, dtype=TRAJ_DTYPE) if traj else None
+        check(lib().cor_trace_plan_results(self.h, tr.ctypes.data if traj else None, cut.ctypes.data),
+              "plan_results")
+        return tr, cut
+
+    def stats(self):
+        st = TraceStats()
+        check(lib().cor_trace_plan_stats(self.h, C.byref(st)), "plan_stats")
+        return st
+
+    def close(self):
+        if self.h:
+            lib().cor_trace_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def trace(models, params, scans, **kw):
+    p = Plan(models, params, scans, **kw)
+    try:
+        p.run()
+        return p.results()
+    finally:
+        p.close()
+
+
+def field(models, params, gsm_m, precision=PREC_FP64):
+    x = np.ascontiguousarray(gsm_m, dtype=np.float64).reshape(-1, 3)
+    out = np.empty_like(x)
+    check(lib().cor_gpu_field(C.byref(models), C.byref(params), x.ctypes.data_as(DP),
+                              out.ctypes.data_as(DP), len(x), precision), "cor_gpu_field")
+    return out
+
+
+def geodetic_to_geocentric(lat_deg, h_km):
+    r0, lat = C.c_double(), C.c_double()
+    check(lib().cor_geodetic_to_geocentric(lat_deg, h_km, C.byref(r0), C.byref(lat)), "geodetic")
+    return r0.value, lat.value
+
+
+def direction_to_geo(lat_deg, lon_deg, zenith_deg, azimuth_deg):
+    t, f = C.c_double(), C.c_double()
+    check(lib().cor_direction_to_geo(lat_deg, lon_deg, zenith_deg, azimuth_deg, C.byref(t), C.byref(f)),
+          "direction")
+    return t.value, f.value
+
+
+def grid_sites(lat_step=5.0, lon_step=5.0, h_km=20.0):
+    """Global vertical-incidence grid: geodetic latitudes -90..90, longitudes 0..360-step."""
+    lats = np.arange(-90.0, 90.0 + 1e-9, lat_step)
+    lons = np.arange(0.0, 360.0 - 1e-9, lon_step)
+    out = []
+    for la in lats:
+        r0, lc = geodetic_to_geocentric(float(la), h_km)
+        for lo in lons:
+            out.append((r0, lc, float(lo)))
+    return out
+
+
+def format_cutoff_lines(scan, traj):
+    """The per-rigidity lines src/trajectory.c:118-122 prints for one scan."""
+    out = []
+    for i, t in enumerate(traj):
+        if t["status"] != ALLOWED:
+            continue
+        rig = scan["rig"] + i * scan["del_"]
+        out.append("%10.6f%15.10f%12.6f%10.3f%10.3f%10.3f%10.3f%12.6f%16.2f" % (
+            rig, t["vc"], t["r"], t["td"], t["fei"], t["ast"], t["asf"], t["time"], t["length_m"] / 1000.))
+    return out

@@ -1,0 +1,562 @@
+// cor_gpu.hip -- the C-ABI of the MI355X tracer (include/cor_gpu.h): per-epoch
+// parameter blocks, scan upload, plan lifetime, kernel launches, the penumbra
+// reduction and the batch field evaluation.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "cor_gpu.h"
+#include "kernels.hpp"
+
+using namespace cor;
+
+namespace {
+
+thread_local char g_hip_error[256] = "";
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+int g_num_cus = 0;
+
+int fail_hip(hipError_t e, const char *what) {
+    snprintf(g_hip_error, sizeof g_hip_error, "%s: %s", what, hipGetErrorString(e));
+    return COR_E_HIP;
+}
+
+#define HIP_TRY(call)                                        \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return fail_hip(e_, #call);    \
+    } while (0)
+
+int ensure_init() {
+    if (g_device >= 0) return COR_OK;
+    return cor_gpu_init(0);
+}
+
+// lib/t96/t96.c:5-21,28-32,54-59: the parameter-only terms of t96_gsm, in the
+// reference's arithmetic (computed here with the host libm, as the reference does).
+template <class T>
+void t96_params(const double *parmod, T96Params<T> &P) {
+    const double pdyn = parmod[0];
+    const double depr = 0.8 * parmod[1] - 13.0 * sqrt(pdyn);
+    const double byimf = parmod[2], bzimf = parmod[3];
+    const double bt = sqrt(byimf * byimf + bzimf * bzimf);
+    double theta = 0.0;
+    if (byimf != 0.0 || bzimf != 0.0) {
+        theta = atan2(byimf, bzimf);
+        if (theta <= 0.0) theta += 6.2831853;
+    }
+    const double ct = cos(theta), st = sin(theta);
+    const double eps = 718.5 * sqrt(pdyn) * bt * sin(theta / 2.);
+    const double xappa = pow(pdyn / 2.0, 0.14);
+    const double am = 70.0 / xappa;
+    const double asq = am * am;
+    const double facteps = eps / 3630.7 - 1.0;
+    const double factpd = sqrt(pdyn / 2.0) - 1.0;
+    const double b1ampl = 0.5790 + 0.4462 * facteps;
+    P.ct = (T)ct;
+    P.st = (T)st;
+    P.oimfy0 = (T)(0.7850 * byimf);
+    P.oimfz0 = (T)(0.7850 * bzimf);
+    P.xappa = (T)xappa;
+    P.xappa3 = (T)(xappa * xappa * xappa);
+    P.am = (T)am;
+    P.asq = (T)asq;
+    P.asq2 = (T)(2.0 * asq);
+    P.asq4 = (T)(4.0 * asq);
+    P.c548 = (T)(5.48 / xappa);
+    P.rcampl = (T)(-1.162 * depr);
+    P.tampl2 = (T)(22.344 + 18.50 * factpd + 2.602 * facteps);
+    P.tampl3 = (T)(6.903 + 5.287 * factpd);
+    P.b1ampl = (T)b1ampl;
+    P.b2ampl = (T)(20.0 * b1ampl);
+    P.rimfampl = (T)(0.7850 * bt);
+}
+
+int model_kinds(const cor_model_set *m, int *intk, int *ext, int *cru) {
+    if (!m) return COR_E_INVALID;
+    if (m->internal != COR_INTERNAL_NONE && m->internal != COR_INTERNAL_IGRF) return COR_E_INVALID;
+    if (m->external < COR_EXTERNAL_NONE || m->external > COR_EXTERNAL_TS05) return COR_E_INVALID;
+    if (m->crustal != COR_CRUSTAL_NONE && m->crustal != COR_CRUSTAL_SH) return COR_E_INVALID;
+    if (m->crustal == COR_CRUSTAL_SH &&
+        (m->sh_nm_max < 2 || m->sh_nm_max > COR_SH_NM_MAX || !m->sh_g || !m->sh_h))
+        return COR_E_INVALID;
+    *intk = m->internal == COR_INTERNAL_IGRF ? INT_IGRF : INT_NONE;
+    *ext = m->external == COR_EXTERNAL_T96 ? EXT_T96 : (m->external == COR_EXTERNAL_TS05 ? EXT_TS05 : EXT_NONE);
+    *cru = m->crustal == COR_CRUSTAL_SH ? CRU_SH : CRU_NONE;
+    return COR_OK;
+}
+
+void build_epoch(const cor_model_set *m, const ModelsParams *p, EpochDev &E) {
+    memset(&E, 0, sizeof E);
+    for (int k = 0; k < NCOEF; k++) {
+        E.G[k] = p->G[k];
+        E.H[k] = p->H[k];
+        E.REC[k] = p->REC[k];
+        E.Gf[k] = (float)p->G[k];
+        E.Hf[k] = (float)p->H[k];
+        E.RECf[k] = (float)p->REC[k];
+    }
+    for (int k = 0; k < 3; k++) {
+        E.A[k] = p->A1[k];
+        E.A[3 + k] = p->A2[k];
+        E.A[6 + k] = p->A3[k];
+    }
+    for (int k = 0; k < 9; k++) E.Af[k] = (float)E.A[k];
+    for (int k = 0; k < 10; k++) E.parmod[k] = p->PARMOD[k];
+    if (m->external == COR_EXTERNAL_T96) {
+        t96_params<double>(p->PARMOD, E.t96d);
+        t96_params<float>(p->PARMOD, E.t96f);
+    }
+    if (m->crustal == COR_CRUSTAL_SH) {
+        const int nm = m->sh_nm_max;
+        E.sh_nm = nm;
+        for (int n = 0; n < nm; n++)
+            for (int k = 0; k < nm; k++) {
+                E.shG[n * SH_NM_MAX + k] = m->sh_g[n * nm + k];
+                E.shH[n * SH_NM_MAX + k] = m->sh_h[n * nm + k];
+                E.shGf[n * SH_NM_MAX + k] = (float)m->sh_g[n * nm + k];
+                E.shHf[n * SH_NM_MAX + k] = (float)m->sh_h[n * nm + k];
+            }
+    }
+}
+
+// Station position and arrival-direction trig as src/trajectory.c:18-31 and
+// src/utils.c:28-41 compute them (host libm, like the reference).
+void build_scan(const cor_scan &s, int64_t first, int32_t n, ScanDev &d) {
+    const double r = s.r0 * DEF_RE;
+    const double th = (90. - s.the0) * DEF_TO_RAD;
+    const double ph = s.fi0 * DEF_TO_RAD;
+    const double sq = r * sin(th);
+    d.pos0[0] = sq * cos(ph);
+    d.pos0[1] = sq * sin(ph);
+    d.pos0[2] = r * cos(th);
+    const double th1 = (90. - s.the1) * DEF_TO_RAD;
+    const double ph1 = s.fi1 * DEF_TO_RAD;
+    d.st1 = sin(th1);
+    d.ct1 = cos(th1);
+    d.sp1 = sin(ph1);
+    d.cp1 = cos(ph1);
+    d.rig = s.rig;
+    d.del = s.del;
+    d.rk = s.rk;
+    d.zn = s.zn;
+    d.nk1 = s.nk1;
+    d.n = n;
+    d.first = first;
+}
+
+// Per scan: src/trajectory.c:49-54,114-115,133-145,156-160.  rmx1/rmx2/rmi are the
+// extreme forbidden/allowed rigidities; since rig + i*del grows with i they are taken
+// at the extreme indices and then evaluated with the reference's double expressions.
+struct ScanStat {
+    unsigned long long steps, attempts;
+};
+
+__global__ void __launch_bounds__(64) penumbra_kernel(const ScanDev *__restrict__ scans,
+                                                      const int32_t *__restrict__ status,
+                                                      const uint32_t *__restrict__ steps,
+                                                      const uint32_t *__restrict__ attempts,
+                                                      cor_cutoff *__restrict__ cut,
+                                                      ScanStat *__restrict__ stat) {
+    const ScanDev &S = scans[blockIdx.x];
+    const int lane = threadIdx.x;
+    int nza = 0, imax1 = -1, imax2 = -1, imin = 0x7fffffff;
+    unsigned long long st = 0, at = 0;
+    for (int i = lane; i < S.n; i += 64) {
+        const int64_t g = S.first + i;
+        const int s = status[g];
+        st += steps[g];
+        at += attempts[g];
+        if (s == ST_ALLOWED) {
+            imin = min(imin, i);
+        } else {
+            nza++;
+            if (s == ST_FORBIDDEN_REENTRY) imax1 = max(imax1, i); else imax2 = max(imax2, i);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nza += __shfl_xor(nza, off);
+        imax1 = max(imax1, __shfl_xor(imax1, off));
+        imax2 = max(imax2, __shfl_xor(imax2, off));
+        imin = min(imin, __shfl_xor(imin, off));
+        st += __shfl_xor(st, off);
+        at += __shfl_xor(at, off);
+    }
+    if (lane == 0) {
+        const double rmx1 = imax1 >= 0 ? fmax(0., S.rig + imax1 * S.del) : 0.;
+        const double rmx2 = imax2 >= 0 ? fmax(0., S.rig + imax2 * S.del) : 0.;
+        const double rmi = imin != 0x7fffffff ? fmin(S.rk, S.rig + imin * S.del) : S.rk;
+        double rmx = fmax(rmx1, rmx2);
+        const double zan = nza - (rmi - S.rig) / S.del;
+        const double rms = rmi + zan * S.del;
+        rmx = rmx + S.del;
+        cor_cutoff c;
+        c.rl = rmi;
+        c.ru = rmx;
+        c.rc = rms;
+        c.n_forbidden = nza;
+        c.n = S.n;
+        cut[blockIdx.x] = c;
+        stat[blockIdx.x].steps = st;
+        stat[blockIdx.x].attempts = at;
+    }
+}
+
+__global__ void reset_head_kernel(unsigned long long *head) { *head = 0ull; }
+
+}  // namespace
+
+struct cor_trace_plan {
+    int intk, ext, cru, prec;
+    const void *kernel;
+    EpochDev *d_epoch;
+    ScanDev *d_scans;
+    std::vector<ScanDev> h_scans;
+    int32_t n_scans;
+    int64_t n_items;
+    uint32_t *d_order;
+    unsigned long long *d_head;
+    TraceOut out;
+    cor_cutoff *d_cut;
+    ScanStat *d_stat;
+    uint64_t step_limit;
+    double tu_angle, cos_tu, mass;
+    int grid, block;
+    hipEvent_t ev[3];
+    bool ran;
+};
+
+extern "C" {
+
+int cor_gpu_init(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        snprintf(g_hip_error, sizeof g_hip_error, "no HIP device");
+        return COR_E_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return COR_E_INVALID;
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        snprintf(g_hip_error, sizeof g_hip_error, "device %d is %s; this library is built for gfx950",
+                 device, prop.gcnArchName);
+        return COR_E_NO_DEVICE;
+    }
+    g_num_cus = prop.multiProcessorCount;
+    if (!g_stream) HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    return COR_OK;
+}
+
+const char *cor_gpu_error_string(int code) {
+    switch (code) {
+        case COR_OK: return "ok";
+        case COR_E_INVALID: return "invalid argument";
+        case COR_E_NO_DEVICE: return "no usable gfx950 device";
+        case COR_E_HIP: return "HIP runtime error";
+        case COR_E_UNSUPPORTED: return "model combination not compiled into the tracer";
+        case COR_E_NOMEM: return "out of memory";
+        default: return "unknown error";
+    }
+}
+
+const char *cor_gpu_last_hip_error(void) { return g_hip_error; }
+
+int cor_gpu_synchronize(void) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    return COR_OK;
+}
+
+int64_t cor_scan_count(const cor_scan *s) {
+    if (!s || !(s->del > 0.) || !isfinite(s->rig) || !isfinite(s->rk)) return COR_E_INVALID;
+    const double q = round((s->rk - s->rig) / s->del);
+    if (!(q < 2.0e9)) return COR_E_INVALID;
+    const int size = (int)q;
+    return size < 0 ? 0 : (int64_t)size + 1;
+}
+
+void cor_trace_plan_destroy(cor_trace_plan *p) {
+    if (!p) return;
+    (void)hipFree(p->d_epoch);
+    (void)hipFree(p->d_scans);
+    (void)hipFree(p->d_order);
+    (void)hipFree(p->d_head);
+    (void)hipFree(p->out.status);
+    (void)hipFree(p->out.steps);
+    (void)hipFree(p->out.attempts);
+    (void)hipFree(p->out.vc);
+    (void)hipFree(p->d_cut);
+    (void)hipFree(p->d_stat);
+    for (int k = 0; k < 3; k++)
+        if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
+    delete p;
+}
+
+int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                          int32_t n_scans, const cor_trace_opts *opts, cor_trace_plan **out) {
+    if (!out) return COR_E_INVALID;
+    *out = nullptr;
+    if (!epoch || !opts || n_scans < 1 || !scans) return COR_E_INVALID;
+    if (opts->precision != COR_PREC_FP64 && opts->precision != COR_PREC_MIXED) return COR_E_INVALID;
+    if (!(opts->particle_mass > 0.) || !(opts->tu_angle > 0.)) return COR_E_INVALID;
+    int rc = ensure_init();
+    if (rc) return rc;
+    int intk, ext, cru;
+    if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
+    const void *kern = opts->precision == COR_PREC_FP64 ? trace_kernel_fp64(intk, ext, cru)
+                                                       : trace_kernel_mixed(intk, ext, cru);
+    if (!kern) return COR_E_UNSUPPORTED;
+
+    cor_trace_plan *p = new cor_trace_plan();   // value-initialised: every pointer null
+    p->intk = intk;
+    p->ext = ext;
+    p->cru = cru;
+    p->prec = opts->precision;
+    p->kernel = kern;
+    p->n_scans = n_scans;
+    p->step_limit = opts->step_limit;
+    p->tu_angle = opts->tu_angle;
+    p->cos_tu = cos(opts->tu_angle);
+    p->mass = opts->particle_mass;
+    p->h_scans.resize(n_scans);
+    int64_t total = 0;
+    for (int32_t s = 0; s < n_scans; s++) {
+        const int64_t n = cor_scan_count(&scans[s]);
+        if (n < 0 || scans[s].nk1 <= 0 || !isfinite(scans[s].zn) || scans[s].zn == 0.) {
+            delete p;
+            return COR_E_INVALID;
+        }
+        build_scan(scans[s], total, (int32_t)n, p->h_scans[s]);
+        total += n;
+    }
+    if (total >= (int64_t)0xffffffffll) {
+        delete p;
+        return COR_E_INVALID;
+    }
+    p->n_items = total;
+
+    EpochDev E;
+    build_epoch(models, epoch, E);
+    const size_t nt = (size_t)std::max<int64_t>(total, 1);
+#define PLAN_TRY(call)                                    \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) {                           \
+            cor_trace_plan_destroy(p);                    \
+            return fail_hip(e_, #call);                   \
+        }                                                 \
+    } while (0)
+    PLAN_TRY(hipMalloc(&p->d_epoch, sizeof(EpochDev)));
+    PLAN_TRY(hipMemcpy(p->d_epoch, &E, sizeof(EpochDev), hipMemcpyHostToDevice));
+    PLAN_TRY(hipMalloc(&p->d_scans, sizeof(ScanDev) * n_scans));
+    PLAN_TRY(hipMemcpy(p->d_scans, p->h_scans.data(), sizeof(ScanDev) * n_scans, hipMemcpyHostToDevice));
+    PLAN_TRY(hipMalloc(&p->d_head, sizeof(unsigned long long)));
+    PLAN_TRY(hipMalloc(&p->out.status, sizeof(int32_t) * nt));
+    PLAN_TRY(hipMalloc(&p->out.steps, sizeof(uint32_t) * nt));
+    PLAN_TRY(hipMalloc(&p->out.attempts, sizeof(uint32_t) * nt));
+    // the eight per-trajectory doubles share one allocation, one array after another
+    PLAN_TRY(hipMalloc(&p->out.vc, sizeof(double) * nt * 8));
+    p->out.r = p->out.vc + nt;
+    p->out.td = p->out.r + nt;
+    p->out.fei = p->out.td + nt;
+    p->out.ast = p->out.fei + nt;
+    p->out.asf = p->out.ast + nt;
+    p->out.time = p->out.asf + nt;
+    p->out.length = p->out.time + nt;
+    PLAN_TRY(hipMemset(p->out.vc, 0, sizeof(double) * nt * 8));
+    PLAN_TRY(hipMalloc(&p->d_cut, sizeof(cor_cutoff) * n_scans));
+    PLAN_TRY(hipMalloc(&p->d_stat, sizeof(ScanStat) * n_scans));
+
+    if (opts->queue_order == 1 && total > 0) {
+        // longest-expected first: trajectories near the lower end of each scan (low
+        // rigidity, often trapped) are issued before the fast-escaping high ones,
+        // interleaved across scans so every scan makes progress.
+        std::vector<uint32_t> order(total);
+        int32_t maxn = 0;
+        for (const ScanDev &d : p->h_scans) maxn = std::max(maxn, d.n);
+        int64_t k = 0;
+        for (int32_t i = 0; i < maxn; i++)
+            for (const ScanDev &d : p->h_scans)
+                if (i < d.n) order[k++] = (uint32_t)(d.first + i);
+        PLAN_TRY(hipMalloc(&p->d_order, sizeof(uint32_t) * total));
+        PLAN_TRY(hipMemcpy(p->d_order, order.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice));
+    }
+
+    // persistent grid: as many resident blocks as the occupancy allows, but no more
+    // blocks than are needed to give every trajectory a lane at the start
+    p->block = 256;
+    int per_cu = 0;
+    PLAN_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p->block, 0));
+    if (per_cu < 1) per_cu = 1;
+    int64_t grid = (int64_t)per_cu * g_num_cus;
+    const int64_t need = (total + 63) / 64;   // one wavefront per 64 trajectories
+    if (need * 64 < (int64_t)p->block * grid) {
+        // small launches: single-wavefront blocks spread over all CUs
+        p->block = 64;
+        grid = std::min<int64_t>(need, (int64_t)per_cu * 4 * g_num_cus);
+    }
+    p->grid = (int)std::max<int64_t>(grid, 1);
+    for (int k = 0; k < 3; k++) PLAN_TRY(hipEventCreate(&p->ev[k]));
+#undef PLAN_TRY
+    *out = p;
+    return COR_OK;
+}
+
+int64_t cor_trace_plan_count(const cor_trace_plan *p) { return p ? p->n_items : COR_E_INVALID; }
+
+int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
+    if (!p) return COR_E_INVALID;
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    TraceArgs a;
+    memset(&a, 0, sizeof a);
+    a.epoch = p->d_epoch;
+    a.scans = p->d_scans;
+    a.n_scans = p->n_scans;
+    a.order = p->d_order;
+    a.id_base = 0;
+    a.n_items = p->n_items;
+    a.head = p->d_head;
+    a.step_limit = p->step_limit;
+    a.tu_angle = p->tu_angle;
+    a.cos_tu = p->cos_tu;
+    a.mass = p->mass;
+    a.out = p->out;
+    reset_head_kernel<<<1, 1, 0, s>>>(p->d_head);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(p->ev[0], s));
+    if (p->n_items > 0) {
+        void *args[] = {&a};
+        HIP_TRY(hipLaunchKernel(p->kernel, dim3(p->grid), dim3(p->block), args, 0, s));
+    }
+    HIP_TRY(hipEventRecord(p->ev[1], s));
+    penumbra_kernel<<<p->n_scans, 64, 0, s>>>(p->d_scans, p->out.status, p->out.steps, p->out.attempts,
+                                              p->d_cut, p->d_stat);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(p->ev[2], s));
+    p->ran = true;
+    return COR_OK;
+}
+
+int cor_trace_plan_sync(cor_trace_plan *p) {
+    if (!p) return COR_E_INVALID;
+    HIP_TRY(hipEventSynchronize(p->ev[2]));
+    return COR_OK;
+}
+
+int cor_trace_plan_results(cor_trace_plan *p, cor_traj *traj, cor_cutoff *cut) {
+    if (!p || !p->ran) return COR_E_INVALID;
+    HIP_TRY(hipEventSynchronize(p->ev[2]));
+    if (cut) HIP_TRY(hipMemcpy(cut, p->d_cut, sizeof(cor_cutoff) * p->n_scans, hipMemcpyDeviceToHost));
+    if (traj && p->n_items > 0) {
+        const size_t n = (size_t)p->n_items;
+        std::vector<int32_t> st(n);
+        std::vector<uint32_t> sp(n), at(n);
+        std::vector<double> d(n * 8);
+        HIP_TRY(hipMemcpy(st.data(), p->out.status, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(sp.data(), p->out.steps, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(at.data(), p->out.attempts, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        // the eight arrays were allocated as one block of n_items-strided arrays
+        const size_t nt = (size_t)std::max<int64_t>(p->n_items, 1);
+        std::vector<double> blk(nt * 8);
+        HIP_TRY(hipMemcpy(blk.data(), p->out.vc, sizeof(double) * nt * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            cor_traj &t = traj[i];
+            t.status = st[i];
+            t.steps = sp[i];
+            t.attempts = at[i];
+            t.pad = 0;
+            t.vc = blk[i];
+            t.r = blk[nt + i];
+            t.td = blk[2 * nt + i];
+            t.fei = blk[3 * nt + i];
+            t.ast = blk[4 * nt + i];
+            t.asf = blk[5 * nt + i];
+            t.time = blk[6 * nt + i];
+            t.length_m = blk[7 * nt + i];
+        }
+    }
+    return COR_OK;
+}
+
+int cor_trace_plan_stats(cor_trace_plan *p, cor_trace_stats *st) {
+    if (!p || !st || !p->ran) return COR_E_INVALID;
+    HIP_TRY(hipEventSynchronize(p->ev[2]));
+    float k = 0.f, t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&k, p->ev[0], p->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t, p->ev[0], p->ev[2]));
+    std::vector<ScanStat> ss(p->n_scans);
+    HIP_TRY(hipMemcpy(ss.data(), p->d_stat, sizeof(ScanStat) * p->n_scans, hipMemcpyDeviceToHost));
+    st->kernel_ms = k;
+    st->total_ms = t;
+    st->steps = 0;
+    st->attempts = 0;
+    for (const ScanStat &x : ss) {
+        st->steps += x.steps;
+        st->attempts += x.attempts;
+    }
+    st->n_traj = p->n_items;
+    st->grid_blocks = p->grid;
+    st->block_threads = p->block;
+    return COR_OK;
+}
+
+int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                  int32_t n_scans, const cor_trace_opts *opts, cor_traj *traj, cor_cutoff *cut) {
+    cor_trace_plan *p = nullptr;
+    int rc = cor_trace_plan_create(models, epoch, scans, n_scans, opts, &p);
+    if (rc) return rc;
+    rc = cor_trace_plan_run(p, nullptr);
+    if (!rc) rc = cor_trace_plan_results(p, traj, cut);
+    cor_trace_plan_destroy(p);
+    return rc;
+}
+
+int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
+                  double *geo_nT, int64_t n, int32_t precision) {
+    if (!epoch || n < 0 || (n > 0 && (!gsm_m || !geo_nT))) return COR_E_INVALID;
+    if (precision != COR_PREC_FP64 && precision != COR_PREC_MIXED) return COR_E_INVALID;
+    int rc = ensure_init();
+    if (rc) return rc;
+    int intk, ext, cru;
+    if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
+    const void *kern = precision == COR_PREC_FP64 ? field_kernel_fp64(intk, ext, cru)
+                                                  : field_kernel_mixed(intk, ext, cru);
+    if (!kern) return COR_E_UNSUPPORTED;
+    if (n == 0) return COR_OK;
+    EpochDev E;
+    build_epoch(models, epoch, E);
+    EpochDev *d_e = nullptr;
+    double *d_in = nullptr, *d_out = nullptr;
+    hipError_t e = hipMalloc(&d_e, sizeof(EpochDev));
+    if (e == hipSuccess) e = hipMalloc(&d_in, sizeof(double) * 3 * n);
+    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 3 * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_e, &E, sizeof(EpochDev), hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in, gsm_m, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) {
+        FieldBatchArgs a;
+        a.epoch = d_e;
+        a.gsm_m = d_in;
+        a.geo_nT = d_out;
+        a.n = n;
+        void *args[] = {&a};
+        e = hipLaunchKernel(kern, dim3((unsigned)((n + 255) / 256)), dim3(256), args, 0, g_stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(geo_nT, d_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    (void)hipFree(d_e);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail_hip(e, "cor_gpu_field");
+    return COR_OK;
+}
+
+}  // extern "C"

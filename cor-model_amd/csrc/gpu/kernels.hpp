@@ -1,0 +1,25 @@
+// kernels.hpp -- kernel tables exported by the per-precision translation units.
+//
+// kernels_fp64.hip is compiled without FMA contraction (the reference's arithmetic);
+// kernels_mixed.hip with contraction and FP32 denormal flushing.  Each exports the
+// tracer instantiations for the supported model sets and a field-batch kernel.
+#pragma once
+#include "dev_types.hpp"
+
+namespace cor {
+
+struct FieldBatchArgs {
+    const EpochDev *epoch;
+    const double *gsm_m;    // n x 3
+    double *geo_nT;         // n x 3
+    int64_t n;
+};
+
+// Returns the kernel for (internal, external, crustal) or nullptr when that model set
+// is not compiled in.
+const void *trace_kernel_fp64(int intk, int ext, int cru);
+const void *trace_kernel_mixed(int intk, int ext, int cru);
+const void *field_kernel_fp64(int intk, int ext, int cru);
+const void *field_kernel_mixed(int intk, int ext, int cru);
+
+}  // namespace cor

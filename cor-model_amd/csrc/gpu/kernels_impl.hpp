@@ -1,0 +1,51 @@
+// kernels_impl.hpp -- instantiation of the tracer and field-batch kernels for one
+// precision; included once by kernels_fp64.hip and once by kernels_mixed.hip.
+#pragma once
+#include "kernels.hpp"
+#include "tracer_dev.hpp"
+
+namespace cor {
+
+// calculate_field over a batch of points (src/models.c:194-207), one point per lane.
+template <int PREC, int INT, int EXT, int CRU>
+__global__ void __launch_bounds__(256) field_batch_kernel(FieldBatchArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const v3<double> gsm = mk3(a.gsm_m[3 * i], a.gsm_m[3 * i + 1], a.gsm_m[3 * i + 2]);
+    v3<double> out;
+    if (PREC == PREC_FP64) {
+        out = field_fp64<INT, EXT, CRU>(a.epoch, gsm);
+    } else {
+        const v3<double> geo = rot_gsm_to_geo(a.epoch->A, gsm);
+        const v3<float> f = field_mixed<INT, EXT, CRU>(a.epoch, geo, gsm);
+        out = mk3((double)f.x, (double)f.y, (double)f.z);
+    }
+    a.geo_nT[3 * i] = out.x;
+    a.geo_nT[3 * i + 1] = out.y;
+    a.geo_nT[3 * i + 2] = out.z;
+}
+
+#define COR_MODEL_SETS(X)                  \
+    X(INT_IGRF, EXT_NONE, CRU_NONE)        \
+    X(INT_IGRF, EXT_T96, CRU_NONE)         \
+    X(INT_NONE, EXT_T96, CRU_NONE)
+
+template <int PREC>
+const void *trace_kernel_table(int intk, int ext, int cru) {
+#define COR_PICK(I, E, C) \
+    if (intk == I && ext == E && cru == C) return (const void *)&trace_kernel<PREC, I, E, C>;
+    COR_MODEL_SETS(COR_PICK)
+#undef COR_PICK
+    return nullptr;
+}
+
+template <int PREC>
+const void *field_kernel_table(int intk, int ext, int cru) {
+#define COR_PICK(I, E, C) \
+    if (intk == I && ext == E && cru == C) return (const void *)&field_batch_kernel<PREC, I, E, C>;
+    COR_MODEL_SETS(COR_PICK)
+#undef COR_PICK
+    return nullptr;
+}
+
+}  // namespace cor

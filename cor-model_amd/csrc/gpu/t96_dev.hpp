@@ -1,0 +1,1123 @@
+// t96_dev.hpp -- Tsyganenko-96 external field (zero tilt) for one lane, templated on
+// the arithmetic type.  GSM position [Re] -> GSM field [nT].
+//
+// Follows lib/t96/t96.c (function-level citations below) with three GPU-specific
+// changes that leave the FP64 arithmetic of every evaluated term unchanged:
+//   * parameter-only terms (lib/t96/t96.c:5-21,28-32,54-59) are computed once per epoch
+//     on the host (T96Params) instead of per call;
+//   * sin/cos(z/r_k) and sqrt(p_i^-2 + r_k^-2) of the Cartesian-harmonic sums are
+//     computed once per k (resp. from constants) instead of once per (i,k) term;
+//   * the region switches of Region 1 (TET0 bands) and Region 2 (xi bands) are
+//     restructured so that each basis (diploop1, condip1, r2outer, r2sheet, r2inner)
+//     is evaluated at most once per lane at a per-lane point, with per-lane weights,
+//     which keeps a wavefront converged instead of serialising the branches.
+// The FP32 instantiation additionally replaces pow() of integer exponents by
+// products and computes 1 - (1 - r12/r22) of the loop integrals as r12/r22.
+#pragma once
+#include "devmath.hpp"
+
+namespace cor {
+
+// Per-epoch constants of t96_gsm (lib/t96/t96.c:5-21,28-32,54-59).
+template <class T>
+struct T96Params {
+    T ct, st;             // cos/sin of the IMF clock angle
+    T oimfy0, oimfz0;     // 0.7850*By, 0.7850*Bz
+    T xappa, xappa3;      // (Pdyn/2)^0.14 and its cube
+    T am, asq, asq2, asq4, c548;   // 70/xappa, am^2, 2 am^2, 4 am^2, 5.48/xappa
+    T rcampl, tampl2, tampl3, b1ampl, b2ampl, rimfampl;
+};
+
+namespace t96 {
+
+// Scheduling fence between independent sub-evaluations: the model is long straight-line
+// code with lots of independent work, and letting the scheduler interleave all of it
+// drives register pressure far past the budget (spills to scratch).  Each fenced block
+// still has its own instruction-level parallelism.
+#define COR_SEQ() __builtin_amdgcn_sched_barrier(0)
+// Materialise a value here: code motion may not sink its computation past this point
+// (an empty asm that claims to read and write the register).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ void pin(T &v) { asm volatile("" : "+v"(v)); }
+#else
+template <class T>
+__host__ __device__ inline void pin(T &) {}
+#endif
+template <class T>
+__device__ __forceinline__ void pin(v3<T> &v) {
+    pin(v.x);
+    pin(v.y);
+    pin(v.z);
+}
+
+// Coefficient tables live in constant memory in both precisions; loops index them with
+// wavefront-uniform counters, so they are read with scalar loads whether or not the
+// compiler unrolls the loop.
+#define COR_CTAB(name, n, ...)                                      \
+    static __constant__ double name##_d[n] = {__VA_ARGS__};        \
+    static __constant__ float name##_f[n] = {__VA_ARGS__};
+__device__ __forceinline__ const double *tab_sel(const double *d, const float *, double) { return d; }
+__device__ __forceinline__ const float *tab_sel(const double *, const float *f, float) { return f; }
+#define CT(name) tab_sel(name##_d, name##_f, T(0))
+
+COR_CTAB(k_dl_c1, 14, -0.911582E-03, -0.376654E-02, -0.727423E-02, -0.270084E-02, -0.123899E-02,
+         -0.154387E-02, -0.340040E-02, -0.191858E-01, -0.518979E-01, 0.635061E-01, 0.440680,
+         -0.396570, -89.5587, 23.2806)
+COR_CTAB(k_dl_xx, 12, -11.0, -7.0, -7.0, -3.0, -3.0, 1.0, 1.0, 1.0, 5.0, 5.0, 9.0, 9.0)
+COR_CTAB(k_dl_yy, 12, 2.0, 0.0, 4.0, 2.0, 6.0, 0.0, 4.0, 8.0, 2.0, 6.0, 0.0, 4.0)
+COR_CTAB(k_cd_c2, 42, 6.04133, 0.305415, 0.606066E-02, 0.128379E-03, -0.179406E-04, 1.41714,
+         -27.2586, -4.28833, -1.30675, 35.5607, 8.95792, 0.961617E-03, -0.801477E-03,
+         -0.782795E-03, -1.65242, -16.5242, -5.33798, 0.424878E-03, 0.331787E-03,
+         -0.704305E-03, 0.844342E-03, 0.953682E-04, 0.886271E-03, 25.1120, 20.9299, 5.14569,
+         -44.1670, -51.0672, -1.87725, 20.2998, 48.7505, -2.97415, 0.174615E-03,
+         -0.789777E-06, 0.686047E-03, 0.460104E-04, -0.345216E-02, 0.221871E-02,
+         0.110078E-01, -0.661373E-02, 0.249201E-02, 0.343978E-01)
+// condip1 dipole positions, already scaled (x, y by 0.08 for i = 2, 4, 5, else 0.4)
+COR_CTAB(k_cd_xd, 9, -10.0 * 0.4, -7.0 * 0.4, -4.0 * 0.08, -4.0 * 0.4, 0.0 * 0.08, 4.0 * 0.08,
+         4.0 * 0.4, 7.0 * 0.4, 10.0 * 0.4)
+COR_CTAB(k_cd_yd, 9, 3.0 * 0.4, 6.0 * 0.4, 3.0 * 0.08, 9.0 * 0.4, 6.0 * 0.08, 3.0 * 0.08,
+         9.0 * 0.4, 6.0 * 0.4, 3.0 * 0.4)
+COR_CTAB(k_cd_zz, 14, 20.0, 20.0, 4.0, 20.0, 4.0, 4.0, 20.0, 20.0, 20.0, 2.0, 3.0, 4.50, 7.0, 10.0)
+COR_CTAB(k_b1s_py, 4, 5.303648988, 10.40368955, 69.65230348, 466.5099509)
+COR_CTAB(k_b1s_pz, 4, 1.645049286, 3.825838190, 11.66675599, 558.9781177)
+COR_CTAB(k_b1s_amp, 16, 1.174198045 + (double)-1.463820502, 4.840161537 + (double)-3.674506864,
+         82.18368896 + (double)-94.94071588, -4122.331796 + (double)4670.278676,
+         -21.54975037 + (double)26.72661293, -72.81365728 + (double)44.09887902,
+         40.08073706 + (double)-51.23563510, 1955.348537 + (double)-1940.971550,
+         794.0496433 + (double)-982.2441344, 1889.837171 + (double)-558.9779727,
+         -1260.543238 + (double)1260.063802, -293.5942373 + (double)344.7250789,
+         -773.7002492 + (double)957.0094135, -1824.143669 + (double)520.7994379,
+         1192.484774 + (double)-1192.184565, 89.15537624 + (double)-98.52042999)
+// tailrc_96 shield sets (lib/t96/t96.c:274-299): per set 9 zero-tilt pair sums
+// a[2j] + a[2j+1], then the y scales p[3] and z scales r[3]
+COR_CTAB(k_shl, 45,
+         -3.087699646 + 3.516259114, 18.81380577 + -13.95772338, -5.497076303 + 0.1712890838,
+         2.392629189 + -2.728020808, -14.79349936 + 11.08738083, 4.388174084 + 0.2492163197E-01,
+         0.7030375685 + -.7966023165, -3.835041334 + 2.642228681, -0.2405352424 + -0.7297705678,
+         12.75434981, 11.37659788, 636.4346279, 1.752483754, 3.604231143, 12.83078674,
+         0.8747515218 + -0.9116821411, 2.209365387 + -2.159059518, -7.059828867 + 5.924671028,
+         -1.916935691 + 1.996707344, -3.877101873 + 3.947666061, 11.38715899 + -8.343210833,
+         1.194109867 + -1.244316975, 3.73895491 + -4.406522465, -20.66884863 + 3.020952989,
+         5.187666221, 6.802472048, 39.13543412, 2.784722096, 6.979576616, 25.71716760,
+         -19091.95061 + -3011.613928, 20582.16203 + 4242.918430, -2377.091102 + -1504.820043,
+         19884.04650 + 2725.150544, -21389.04845 + -3990.475093, 2401.610097 + 1548.171792,
+         -946.5493963 + 490.1528941, 986.9156625 + -489.3265930, -67.99278499 + 8.711175710,
+         19.69877970, 20.30095680, 86.45407420, 22.50403727, 23.41617329, 48.48140573)
+// r2sheet coefficients [component x,y,z][latitude term][phi harmonic][t factor]
+COR_CTAB(k_r2s, 240,
+         8.07190, -7.39582, -7.62341, 0.684671, -13.5672, 11.6681, 13.1154, -0.890217,
+         7.78726, -5.38346, -8.08738, 0.609385, -2.70410, 3.53741, 3.15549, -1.11069,
+         -8.47555, 0.278122, 2.73514, 4.55625, 13.1134, 1.15848, -3.52648, -8.24698,
+         -6.85710, -2.81369, 2.03795, 4.64383, 2.49309, -1.22041, -1.67432, -0.422526,
+         -5.39796, 7.10326, 5.53730, -13.1918, 4.67853, -7.60329, -2.53066, 7.76338,
+         5.60165, 5.34816, -4.56441, 7.05976, -2.62723, -0.529078, 1.42019, -2.93919,
+         55.6338, -1.55181, 39.8311, -80.6561, -46.9655, 32.8925, -6.32296, 19.7841,
+         124.731, 10.4347, -30.7581, 102.680, -47.4037, -3.31278, 9.37141, -50.0268,
+         -533.319, 110.426, 1000.20, -1051.40, 1619.48, 589.855, -1462.73, 1087.10,
+         -1994.73, -1654.12, 1263.33, -260.210, 1424.84, 1255.71, -956.733, 219.946,
+         -9.08427, 10.6777, 10.3288, -0.969987, 6.45257, -8.42508, -7.97464, 1.41996,
+         -1.92490, 3.93575, 2.83283, -1.48621, 0.244033, -0.757941, -0.386557, 0.344566,
+         9.56674, -2.5365, -3.32916, -5.86712, -6.19625, 1.83879, 2.52772, 4.34417,
+         1.87268, -2.13213, -1.69134, -0.176379, -0.261359, 0.566419, 0.3138, -0.134699,
+         -3.83086, -8.4154, 4.77005, -9.31479, 37.5715, 19.3992, -17.9582, 36.4604,
+         -14.9993, -3.1442, 6.17409, -15.5519, 2.28621, -0.00891549, -0.462912, 2.47314,
+         41.7555, 208.614, -45.7861, -77.8687, 239.357, -67.9226, 66.8743, 238.534,
+         -112.136, 16.2069, -40.4706, -134.328, 21.56, -0.201725, 2.21, 32.5855,
+         -108.217, -1005.98, 585.753, 323.668, -817.056, 235.750, -560.965, -576.892,
+         684.193, 85.0275, 168.394, 477.776, -289.253, -123.216, 75.6501, -178.605,
+         1167.61, -917.782, -1253.2, -274.128, -1538.75, 1257.62, 1745.07, 113.479,
+         393.326, -426.858, -641.1, 190.833, -29.9435, -1.04881, 117.125, -25.7663,
+         -1168.16, 910.247, 1239.31, 289.515, 1540.56, -1248.29, -1727.61, -131.785,
+         -394.577, 426.163, 637.422, -187.965, 30.0348, 0.221898, -116.68, 26.0291,
+         12.6804, 4.84091, 1.18166, -2.75946, -17.9822, -6.80357, -1.47134, 3.02266,
+         4.79648, 0.665255, -0.256229, -0.0857282, -0.588997, 0.0634812, 0.164303, -0.15285,
+         22.2524, -22.4376, -3.85595, 6.07625, -105.959, -41.6698, 0.378615, 1.55958,
+         44.3981, 18.8521, 3.19466, 5.89142, -8.63227, -2.36418, -1.027, -2.31515,
+         1035.38, 2040.66, -131.881, -744.533, -3274.93, -4845.61, 482.438, 1567.43,
+         1354.02, 2040.47, -151.653, -845.012, -111.723, -265.343, -26.1171, 216.632)
+
+template <class T>
+__device__ __forceinline__ v3<T> dipole(T x, T y, T z) {
+    // lib/t96/t96.c:1645-1656 (reference form of the dipole)
+    const T p = x * x, u = z * z, v = T(3.0) * z * x, t = y * y;
+    const T r = Dm<T>::sqrt(p + t + u);
+    const T q = T(30574.0) / (r * r * r * r * r);
+    return mk3<T>(q * ((t + u - T(2.0) * p) - v), T(-3.0) * y * q * z, q * (p + t - T(2.0) * u));
+}
+
+// lib/t96/t96.c:212-234
+template <class T>
+__device__ __forceinline__ void bessel01(T x, T *j0, T *j1) {
+    if (Dm<T>::fabs(x) < T(3.0)) {
+        const T q = (x / T(3.0)) * (x / T(3.0));
+        *j0 = T(1.0) - q * (T(2.2499997) - q * (T(1.2656208) - q * (T(0.3163866) - q * (T(0.0444479) - q * (T(0.0039444) - q * T(0.00021))))));
+        const T j1x = T(0.5) - q * (T(0.56249985) - q * (T(0.21093573) - q * (T(0.03954289) - q * (T(0.00443319) - q * (T(0.00031761) - q * T(0.00001109))))));
+        *j1 = j1x * x;
+    } else {
+        const T u = T(3.0) / x;
+        const T sx = Dm<T>::sqrt(x);
+        const T f0 = T(0.79788456) - u * (T(0.00000077) + u * (T(0.00552740) + u * (T(0.00009512) - u * (T(0.00137237) - u * (T(0.00072805) - u * T(0.00014476))))));
+        const T t0 = x - T(0.78539816) - u * (T(0.04166397) + u * (T(0.00003954) - u * (T(0.00262573) - u * (T(0.00054125) + u * (T(0.00029333) - u * T(0.00013558))))));
+        *j0 = f0 / sx * Dm<T>::cos(t0);
+        const T f1 = T(0.79788456) + u * (T(0.00000156) + u * (T(0.01659667) + u * (T(0.00017105) - u * (T(0.00249511) - u * (T(0.00113653) - u * T(0.00020033))))));
+        const T t1 = x - T(2.35619449) + u * (T(0.12499612) + u * (T(0.0000565) - u * (T(0.00637879) - u * (T(0.00074348) + u * (T(0.00079824) - u * T(0.00029166))))));
+        *j1 = f1 / sx * Dm<T>::cos(t1);
+    }
+}
+
+// lib/t96/t96.c:113-163
+template <class T>
+__device__ __forceinline__ v3<T> cylharm(T x, T y, T z) {
+    const T a[12] = {T(0.24777), T(-27.003), T(-0.46815), T(7.0637), T(-1.5918), T(-0.90317E-01),
+                     T(57.522), T(13.757), T(2.0100), T(10.458), T(4.5798), T(2.1695)};
+    T rho = Dm<T>::sqrt(y * y + z * z);
+    T sinfi, cosfi;
+    if (rho < T(1.0e-8)) {
+        sinfi = T(1.0);
+        cosfi = T(0.0);
+        rho = T(1.0e-8);
+    } else {
+        sinfi = z / rho;
+        cosfi = y / rho;
+    }
+    const T sinfi2 = sinfi * sinfi;
+    const T si2co2 = sinfi2 - cosfi * cosfi;
+    v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const T dz = rho / a[i + 6];
+        T j0, j1;
+        bessel01(dz, &j0, &j1);
+        const T e = Dm<T>::exp(x / a[i + 6]);
+        b.x -= a[i] * j1 * e * sinfi;
+        b.y += a[i] * (T(2.0) * j1 / dz - j0) * e * sinfi * cosfi;
+        b.z += a[i] * (j1 / dz * si2co2 - j0 * sinfi2) * e;
+    }
+#pragma unroll
+    for (int i = 3; i < 6; i++) {
+        const T dz = rho / a[i + 6];
+        const T xk = x / a[i + 6];
+        T j0, j1;
+        bessel01(dz, &j0, &j1);
+        const T e = Dm<T>::exp(xk);
+        const T brho = (xk * j0 - (dz * dz + xk - T(1.0)) * j1 / dz) * e * sinfi;
+        const T bphi = (j0 + j1 / dz * (xk - T(1.0))) * e * cosfi;
+        b.x += a[i] * (dz * j0 + xk * j1) * e * sinfi;
+        b.y += a[i] * (brho * cosfi - bphi * sinfi);
+        b.z += a[i] * (brho * sinfi + bphi * cosfi);
+    }
+    return b;
+}
+
+// lib/t96/t96.c:236-271 (sin/cos of z*rr[k] hoisted out of the i loop)
+template <class T>
+__device__ __forceinline__ v3<T> intercon(T x, T y, T z) {
+    const T a[9] = {T(-8.411078731), T(5932254.951), T(-9073284.93), T(-11.68794634),
+                    T(6027598.824), T(-9218378.368), T(-6.508798398), T(-11824.42793),
+                    T(18015.66212)};
+    const T rp[3] = {T(0.12503844191332988), T(0.071597395204476152), T(0.011080976889832743)};
+    const T rr[3] = {T(0.059675527522452816), T(0.00098459526668325296), T(0.00064371058922777988)};
+    T sz[3], cz[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) Dm<T>::sincos(z * rr[k], &sz[k], &cz[k]);
+    v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        T sy, cy;
+        Dm<T>::sincos(y * rp[i], &sy, &cy);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const T sq = Dm<T>::sqrt(rp[i] * rp[i] + rr[k] * rr[k]);
+            const T e = Dm<T>::exp(x * sq);
+            const T c = a[3 * i + k];
+            b.x += c * (-sq * e * cy * sz[k]);
+            b.y += c * (rp[i] * e * sy * sz[k]);
+            b.z += c * (-rr[k] * e * cy * cz[k]);
+        }
+    }
+    return b;
+}
+
+// Conical current-sheet kernel shared by ringcurr_96 (lib/t96/t96.c:342-375) and
+// taildisk_t96 (:400-436): s1, s2 distances to the ring, and the derivatives of the
+// vector-potential factor AS with respect to s1, s2.
+template <class T>
+__device__ __forceinline__ void cone_terms(T zb, T rhos, T bi, T *as, T *das1, T *das2,
+                                           T *ds1dz, T *ds2dz, T *ds1dr, T *ds2dr) {
+    const T rp = rhos + bi, rm = rhos - bi;
+    const T zb2 = zb * zb, b2 = T(2.0) * bi;
+    const T s1 = Dm<T>::sqrt(zb2 + rp * rp), s2 = Dm<T>::sqrt(zb2 + rm * rm);
+    *ds1dz = zb / s1;
+    *ds2dz = zb / s2;
+    *ds1dr = rp / s1;
+    *ds2dr = rm / s2;
+    const T s12 = s1 * s2, sp = s1 + s2, sp2 = sp * sp;
+    const T fac1 = Dm<T>::sqrt(sp2 - b2 * b2);
+    *as = fac1 / (s12 * sp2);
+    const T term1 = T(1.0) / (s12 * sp * fac1);
+    const T fac2 = *as / sp2;
+    *das1 = term1 - fac2 / s1 * (s2 * s2 + s1 * (T(3.0) * s1 + T(4.0) * s2));
+    *das2 = term1 - fac2 / s2 * (s1 * s1 + s2 * (T(3.0) * s2 + T(4.0) * s1));
+}
+
+// lib/t96/t96.c:322-381
+template <class T>
+__device__ __forceinline__ v3<T> ringcurr(T x, T y, T z) {
+    const T f[2] = {T(569.895366), T(-1603.386993)};
+    const T beta[2] = {T(2.722188), T(3.766875)};
+    const T dzs = Dm<T>::sqrt(z * z + T(4.0));
+    const T rhos = Dm<T>::sqrt(x * x + y * y);
+    const T ddz_dz = z / dzs;
+    T drx = T(0.0), dry = (y >= 0) ? T(1.0) : T(-1.0);
+    if (rhos >= T(1.0e-5)) {
+        drx = x / rhos;
+        dry = y / rhos;
+    }
+    v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        T as, d1, d2, s1z, s2z, s1r, s2r;
+        cone_terms(dzs + beta[i], rhos, beta[i], &as, &d1, &d2, &s1z, &s2z, &s1r, &s2r);
+        const T dasx = d1 * (s1r * drx) + d2 * (s2r * drx);
+        const T dasy = d1 * (s1r * dry) + d2 * (s2r * dry);
+        const T dasz = d1 * (s1z * ddz_dz) + d2 * (s2z * ddz_dz);
+        b.x += -f[i] * x * dasz;
+        b.y -= f[i] * y * dasz;
+        b.z += f[i] * ((T(2.0) * as + y * dasy) + x * dasx);
+    }
+    return b;
+}
+
+// lib/t96/t96.c:383-443
+template <class T>
+__device__ __forceinline__ v3<T> taildisk(T x, T y, T z) {
+    const T f[4] = {T(-745796.7338), T(1176470.141), T(-444610.529), T(-57508.01028)};
+    const T beta[4] = {T(7.9250000), T(8.0850000), T(8.4712500), T(27.89500)};
+    const T y20 = y / T(20.0);
+    const T dd = T(2.0) + T(10.0) * (y20 * y20);
+    const T dzs = Dm<T>::sqrt(z * z + dd * dd);
+    const T ddz_dy = (dd * T(10.0) * y * T(0.005)) / dzs;
+    const T ddz_dz = z / dzs;
+    const T xs = x - T(4.5);
+    const T rhos = Dm<T>::sqrt(xs * xs + y * y);
+    T drx = T(0.0), dry = (y >= 0) ? T(1.0) : T(-1.0);
+    if (rhos >= T(1.0e-5)) {
+        drx = xs / rhos;
+        dry = y / rhos;
+    }
+    v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        T as, d1, d2, s1z, s2z, s1r, s2r;
+        cone_terms(dzs + beta[i], rhos, beta[i], &as, &d1, &d2, &s1z, &s2z, &s1r, &s2r);
+        const T dasx = d1 * (s1r * drx) + d2 * (s2r * drx);
+        const T dasy = d1 * (s1z * ddz_dy + s1r * dry) + d2 * (s2z * ddz_dy + s2r * dry);
+        const T dasz = d1 * (s1z * ddz_dz) + d2 * (s2z * ddz_dz);
+        b.x += f[i] * ((T(4.5) - x) * dasz);
+        b.y -= f[i] * y * dasz;
+        b.z += f[i] * ((T(2.0) * as + y * dasy) + xs * dasx);
+    }
+    return b;
+}
+
+// lib/t96/t96.c:445-499
+template <class T>
+__device__ __forceinline__ void tail87(T x, T z, T *bx, T *bz) {
+    const T zp = z - T(40.0), zm = z + T(40.0);
+    const T xnx = T(-10.0) - x, xnx2 = xnx * xnx;
+    const T xc1 = x + T(1.261), xc2 = x + T(0.663);
+    const T xc22 = xc2 * xc2, xr2 = xc2 * T(-0.1071), xc12 = xc1 * xc1;
+    const T d2 = T(3.0) * T(3.0);
+    const T b20 = z * z + d2, b2p = zp * zp + d2, b2m = zm * zm + d2;
+    const T b = Dm<T>::sqrt(b20), bp = Dm<T>::sqrt(b2p), bm = Dm<T>::sqrt(b2m);
+    const T xa1 = xc12 + b20, xap1 = xc12 + b2p, xam1 = xc12 + b2m;
+    const T xa2 = T(1.) / (xc22 + b20), xap2 = T(1.) / (xc22 + b2p), xam2 = T(1.) / (xc22 + b2m);
+    const T xna = xnx2 + b20, xnap = xnx2 + b2p, xnam = xnx2 + b2m;
+    const T ff = b20 - xc22, fp = b2p - xc22, fm = b2m - xc22;
+    const T ln1 = Dm<T>::log(T(76.37) / xna), lnp1 = Dm<T>::log(T(76.37) / xnap),
+            lnm1 = Dm<T>::log(T(76.37) / xnam);
+    const T ln2 = ln1 + T(0.13238005), lnp2 = lnp1 + T(0.13238005), lnm2 = lnm1 + T(0.13238005);
+    const T aln = T(0.25) * (lnp1 + lnm1 - T(2.) * ln1);
+    const T s0 = (Dm<T>::atan(xnx / b) + T(1.5707963)) / b;
+    const T s0p = (Dm<T>::atan(xnx / bp) + T(1.5707963)) / bp;
+    const T s0m = (Dm<T>::atan(xnx / bm) + T(1.5707963)) / bm;
+    const T s1 = (ln1 * T(.5) + xc1 * s0) / xa1;
+    const T s1p = (lnp1 * T(.5) + xc1 * s0p) / xap1;
+    const T s1m = (lnm1 * T(.5) + xc1 * s0m) / xam1;
+    const T s2 = (xc2 * xa2 * ln2 + T(0.1071) - ff * xa2 * s0) * xa2;
+    const T s2p = (xc2 * xap2 * lnp2 + T(0.1071) - fp * xap2 * s0p) * xap2;
+    const T s2m = (xc2 * xam2 * lnm2 + T(0.1071) - fm * xam2 * s0m) * xam2;
+    const T g1 = (b20 * s0 - T(0.5) * xc1 * ln1) / xa1;
+    const T g1p = (b2p * s0p - T(0.5) * xc1 * lnp1) / xap1;
+    const T g1m = (b2m * s0m - T(0.5) * xc1 * lnm1) / xam1;
+    const T g2 = ((T(0.5) * ff * ln2 + T(2.) * s0 * b20 * xc2) * xa2 + xr2) * xa2;
+    const T g2p = ((T(0.5) * fp * lnp2 + T(2.) * s0p * b2p * xc2) * xap2 + xr2) * xap2;
+    const T g2m = ((T(0.5) * fm * lnm2 + T(2.) * s0m * b2m * xc2) * xam2 + xr2) * xam2;
+    *bx = T(0.391734) * (z * s0 - T(0.5) * (zp * s0p + zm * s0m)) + T(5.89715) * (z * s1 - T(0.5) * (zp * s1p + zm * s1m)) + T(24.6833) * (z * s2 - T(0.5) * (zp * s2p + zm * s2m));
+    *bz = T(0.391734) * aln + T(5.89715) * (g1 - T(0.5) * (g1p + g1m)) + T(24.6833) * (g2 - T(0.5) * (g2p + g2m));
+}
+
+// lib/t96/t96.c:501-617 for coefficient set `set` of k_shl (0: ring current, 1: tail
+// mode 2, 2: tail mode 3).
+template <class T>
+__device__ __forceinline__ v3<T> shlcar3x3(int set, T x, T y, T z) {
+    const T *tab = CT(k_shl) + 15 * set;
+    T szr[3], czr[3], rinv2[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const T r = tab[12 + k];
+        Dm<T>::sincos(z / r, &szr[k], &czr[k]);
+        rinv2[k] = T(1.0) / (r * r);
+    }
+    v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const T p = tab[9 + i];
+        const T pp = T(1.0) / (p * p);
+        T sy, cy;
+        Dm<T>::sincos(y / p, &sy, &cy);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const T r = tab[12 + k];
+            const T sq = Dm<T>::sqrt(pp + rinv2[k]);
+            const T e = Dm<T>::exp(x * sq);
+            const T c = tab[3 * i + k];
+            b.x += (-sq * e * cy * szr[k]) * c;
+            b.y += (e / p * sy * szr[k]) * c;
+            b.z += (-e / r * cy * czr[k]) * c;
+        }
+    }
+    return b;
+}
+
+// lib/t96/t96.c:854-881
+template <class T>
+__device__ __forceinline__ v3<T> circle(T x, T y, T z, T rl) {
+    const T rho2 = x * x + y * y;
+    const T rho = Dm<T>::sqrt(rho2);
+    const T rr = rho + rl;
+    const T r22 = z * z + rr * rr;
+    const T r2 = Dm<T>::sqrt(r22);
+    const T r12 = r22 - T(4.0) * rho * rl;
+    const T r32 = T(0.5) * (r12 + r22);
+    T xk2s;
+    if (Dm<T>::fp64) {
+        const T xk2 = T(1.0) - r12 / r22;
+        xk2s = T(1.0) - xk2;
+    } else {
+        xk2s = r12 / r22;   // same quantity without the 1 - (1 - q) cancellation
+    }
+    const T dl = Dm<T>::log(T(1.0) / xk2s);
+    const T kk = T(1.38629436112) + xk2s * (T(0.09666344259) + xk2s * (T(0.03590092383) +
+                 xk2s * (T(0.03742563713) + xk2s * T(0.01451196212)))) + dl *
+                 (T(0.5) + xk2s * (T(0.12498593597) + xk2s * (T(0.06880248576) +
+                 xk2s * (T(0.03328355346) + xk2s * T(0.00441787012)))));
+    const T ee = T(1.0) + xk2s * (T(0.44325141463) + xk2s * (T(0.0626060122) + xk2s *
+                 (T(0.04757383546) + xk2s * T(0.01736506451)))) + dl *
+                 xk2s * (T(0.2499836831) + xk2s * (T(0.09200180037) + xk2s *
+                 (T(0.04069697526) + xk2s * T(0.00526449639))));
+    T brho;
+    if (rho > T(1e-6)) {
+        brho = z / (rho2 * r2) * (r32 / r12 * ee - kk);
+    } else {
+        brho = T(3.141592654) * rl / r2 * (rl - rho) / r12 * z / (r32 - rho2);
+    }
+    v3<T> out = mk3<T>(brho * x, brho * y, (kk - ee * (r32 - T(2.0) * rl * rl) / r12) / r2);
+    pin(out);
+    return out;
+}
+
+// lib/t96/t96.c:883-898
+template <class T>
+__device__ __forceinline__ v3<T> crosslp(T x, T y, T z, T xc, T rl, T cal, T sal) {
+    const T y1 = y * cal - z * sal, z1 = y * sal + z * cal;
+    const T y2 = y * cal + z * sal, z2 = -y * sal + z * cal;
+    const v3<T> b1 = circle(x - xc, y1, z1, rl);
+    COR_SEQ();
+    const v3<T> b2 = circle(x - xc, y2, z2, rl);
+    COR_SEQ();
+    return mk3<T>(b1.x + b2.x, (b1.y + b2.y) * cal + (b1.z - b2.z) * sal,
+                  -(b1.y - b2.y) * sal + (b1.z + b2.z) * cal);
+}
+
+// lib/t96/t96.c:900-918
+template <class T>
+__device__ __forceinline__ void dipxyz(T x, T y, T z, v3<T> *mx, v3<T> *my, v3<T> *mz) {
+    const T x2 = x * x, y2 = y * y, z2 = z * z;
+    const T r2 = x2 + y2 + z2;
+    const T q = T(30574.0) / (r2 * r2 * Dm<T>::sqrt(r2));
+    const T q3 = T(3.0) * q;
+    mx->x = q * (T(3.0) * x2 - r2);
+    mx->y = q3 * x * y;
+    mx->z = q3 * x * z;
+    my->x = mx->y;
+    my->y = q * (T(3.0) * y2 - r2);
+    my->z = q3 * y * z;
+    mz->x = mx->z;
+    mz->y = my->z;
+    mz->z = q * (T(3.0) * z2 - r2);
+}
+
+// lib/t96/t96.c:809-852 contracted with C1 (lib/t96/t96.c:620-625) as birk1tot_02
+// sums it: sum_i c1[i] * d[.][i] in index order.
+template <class T>
+__device__ __forceinline__ v3<T> diploop_sum(T x, T y, T z) {
+    const T *c1 = CT(k_dl_c1), *xx = CT(k_dl_xx), *yy = CT(k_dl_yy);
+    const T z2 = z * z;
+    v3<T> b = mk3<T>(0, 0, 0);
+    for (int i = 0; i < 12; i++) {
+        const T yd = yy[i] * T(0.945719);
+        const T xd = xx[i] * T(1.12541);
+        const T dx = x - xd;
+        T dy = y - yd;
+        T r2 = dx * dx + dy * dy + z2;
+        T q = T(30574.0) / (r2 * r2 * Dm<T>::sqrt(r2));
+        T q3 = T(3.0) * q;
+        const T b1x = q3 * dx * z, b1y = q3 * dy * z, b1z = q * (T(3.0) * z2 - r2);
+        T b2x = T(0.0), b2y = T(0.0), b2z = T(0.0);
+        if (yy[i] != T(0.0)) {   // fabs(yd) > 1e-10 holds exactly when yy != 0
+            dy = y + yd;
+            r2 = dx * dx + dy * dy + z2;
+            q = T(30574.0) / (r2 * r2 * Dm<T>::sqrt(r2));
+            q3 = T(3.0) * q;
+            b2x = q3 * dx * z;
+            b2y = q3 * dy * z;
+            b2z = q * (T(3.0) * z2 - r2);
+        }
+        b.x += c1[i] * (b1x + b2x);
+        b.y += c1[i] * (b1y + b2y);
+        b.z += c1[i] * (b1z + b2z);
+        COR_SEQ();
+    }
+    const v3<T> oct = crosslp(x, y, z, T(2.28397), T(1.86106), T(0.5327834519503802), T(0.8462516134861054));
+    b.x += c1[12] * oct.x;
+    b.y += c1[12] * oct.y;
+    b.z += c1[12] * oct.z;
+    const v3<T> cl = circle(x + T(5.60831), y, z, T(7.83281));
+    b.x += c1[13] * cl.x;
+    b.y += c1[13] * cl.y;
+    b.z += c1[13] * cl.z;
+    return b;
+}
+
+// lib/t96/t96.c:920-1028 contracted with C2 (lib/t96/t96.c:627-637).  The basis
+// functions are produced in the index order 0..41 of the reference's d[3][42] and
+// contracted with c2 as they are produced (same sum order).
+template <class T>
+__device__ __forceinline__ v3<T> condip_sum(T x, T y, T z) {
+    const T *c2 = CT(k_cd_c2), *xdt = CT(k_cd_xd), *ydt = CT(k_cd_yd), *zz = CT(k_cd_zz);
+    v3<T> b = mk3<T>(0, 0, 0);
+#define COR_ACC(I, VX, VY, VZ)  \
+    {                           \
+        b.x += c2[I] * (VX);    \
+        b.y += c2[I] * (VY);    \
+        b.z += c2[I] * (VZ);    \
+    }
+    const T xsm = x + T(0.16);
+    const T ro2 = xsm * xsm + y * y;
+    const T ro = Dm<T>::sqrt(ro2);
+    const T cf0 = xsm / ro, sf0 = y / ro;
+    const T r2 = ro2 + z * z;
+    const T r = Dm<T>::sqrt(r2);
+    const T c = z / r, s = ro / r;
+    const T ch = Dm<T>::sqrt(T(0.5) * (T(1.0) + c)), sh = Dm<T>::sqrt(T(0.5) * (T(1.0) - c));
+    const T tnh = sh / ch, cnh = T(1.0) / tnh;
+    // conical harmonics m = 1..5 with cos/sin(m phi) by the reference's recurrence
+    T cfm = cf0, sfm = sf0;
+    T tp = T(1.0), cp = T(1.0);     // tnh^m, cnh^m (FP32 path; the FP64 path calls pow)
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        if (m == 1) {
+            cfm = cf0 * cf0 - sf0 * sf0;
+            sfm = T(2.0) * sf0 * cf0;
+        } else if (m > 1) {
+            const T cn = cfm * cf0 - sfm * sf0;
+            sfm = sfm * cf0 + cfm * sf0;
+            cfm = cn;
+        }
+        const T m1 = T(m + 1);
+        T tm, cmm, tm1, cm1;
+        if (Dm<T>::fp64) {
+            tm1 = Dm<T>::powi(tnh, m + 1);
+            cm1 = Dm<T>::powi(cnh, m + 1);
+            tm = Dm<T>::powi(tnh, m);
+            cmm = Dm<T>::powi(cnh, m);
+        } else {
+            tm = tp;
+            cmm = cp;
+            tp = tp * tnh;
+            cp = cp * cnh;
+            tm1 = tp;
+            cm1 = cp;
+        }
+        const T bt = m1 * cfm / (r * s) * (tm1 + cm1);
+        const T bf = T(-0.5) * m1 * sfm / r * (tm / (ch * ch) - cmm / (sh * sh));
+        COR_ACC(m, bt * c * cf0 - bf * sf0, bt * c * sf0 + bf * cf0, -bt * s)
+    }
+    for (int i = 0; i < 9; i++) {
+        const T xd = xdt[i], yd = ydt[i], zd = zz[i];
+        v3<T> m1x, m1y, m1z, m2x, m2y, m2z, m3x, m3y, m3z, m4x, m4y, m4z;
+        dipxyz(x - xd, y - yd, z - zd, &m1x, &m1y, &m1z);
+        dipxyz(x - xd, y + yd, z - zd, &m2x, &m2y, &m2z);
+        dipxyz(x - xd, y - yd, z + zd, &m3x, &m3y, &m3z);
+        dipxyz(x - xd, y + yd, z + zd, &m4x, &m4y, &m4z);
+        const int ix = (i + 1) * 3 + 2;
+        COR_ACC(ix, m1x.x + m2x.x - m3x.x - m4x.x, m1x.y + m2x.y - m3x.y - m4x.y,
+                m1x.z + m2x.z - m3x.z - m4x.z)
+        COR_ACC(ix + 1, m1y.x - m2y.x - m3y.x + m4y.x, m1y.y - m2y.y - m3y.y + m4y.y,
+                m1y.z - m2y.z - m3y.z + m4y.z)
+        COR_ACC(ix + 2, m1z.x + m2z.x + m3z.x + m4z.x, m1z.y + m2z.y + m3z.y + m4z.y,
+                m1z.z + m2z.z + m3z.z + m4z.z)
+        COR_SEQ();
+    }
+    for (int i = 1; i <= 5; i++) {
+        const T zd = zz[i + 8];
+        v3<T> m1x, m1y, m1z, m2x, m2y, m2z;
+        dipxyz(x, y, z - zd, &m1x, &m1y, &m1z);
+        dipxyz(x, y, z + zd, &m2x, &m2y, &m2z);
+        const int ix = 30 + i * 2;
+        COR_ACC(ix, m1x.x - m2x.x, m1x.y - m2x.y, m1x.z - m2x.z)
+        COR_ACC(ix + 1, m1z.x + m2z.x, m1z.y + m2z.y, m1z.z + m2z.z)
+    }
+#undef COR_ACC
+    return b;
+}
+
+// lib/t96/t96.c:1030-1194 (4x4 harmonics; sin/cos(z r_k) once per k)
+template <class T>
+__device__ __forceinline__ v3<T> birk1shld(T x, T y, T z) {
+    const T *py = CT(k_b1s_py), *pz = CT(k_b1s_pz), *amp = CT(k_b1s_amp);
+    T rk[4], rk2[4], szk[4], czk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        rk[k] = T(1.0) / pz[k];
+        Dm<T>::sincos(z * rk[k], &szk[k], &czk[k]);
+        rk2[k] = rk[k] * rk[k];
+    }
+    v3<T> b = mk3<T>(0, 0, 0);
+    for (int i = 0; i < 4; i++) {
+        const T rp = T(1.0) / py[i];
+        const T rp2 = rp * rp;
+        T sy, cy;
+        Dm<T>::sincos(rp * y, &sy, &cy);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const T sq = Dm<T>::sqrt(rp2 + rk2[k]);
+            const T e = Dm<T>::exp(x * sq);
+            const T c = amp[4 * i + k];
+            b.x += (-sq * e * cy * szk[k]) * c;
+            b.y += (rp * e * sy * szk[k]) * c;
+            b.z += (-rk[k] * e * cy * czk[k]) * c;
+        }
+    }
+    return b;
+}
+
+// lib/t96/t96.c:619-807.  Which Region-1 expansion applies depends on the
+// dipole-mapped colatitude tet0: the loop basis poleward of the noon/midnight band
+// edges, the conical basis between them, and a linear blend (by distance along the
+// segment between the two band-edge mapped points) inside the two 4-degree bands.
+// Each basis is evaluated once per lane at its own point; the blend keeps the
+// reference's "first point, second point" order of the weighted sum.
+template <class T>
+__device__ __forceinline__ v3<T> birk1tot(T x, T y, T z) {
+    const T tnoonn = T(12.0 * 0.01745329);
+    const T tnoons = T(3.141592654 - 12.0 * 0.01745329);
+    const T r2 = x * x + y * y + z * z;
+    const T r = Dm<T>::sqrt(r2);
+    const T r3 = r * r2;
+    T pas = T(0.0);
+    if (x != T(0.0) || y != T(0.0)) pas = Dm<T>::atan2(y, x);
+    const T tas = Dm<T>::atan2(Dm<T>::sqrt(x * x + y * y), z);
+    const T stas = Dm<T>::sin(tas);
+    T tet0 = Dm<T>::asin(stas / Dm<T>::pow((stas * stas * stas * stas * stas * stas) * (T(1.0) - r3) + r3, T(0.1666666667)));
+    if (tas > T(1.5707963)) tet0 = T(3.141592654) - tet0;
+    const T sp2 = Dm<T>::sin(pas * T(0.5));
+    const T dtet = T(8.0 * 0.01745329) * (sp2 * sp2);
+    const T tetr1n = tnoonn + dtet, tetr1s = tnoons - dtet;
+    const T t01n = tetr1n - T(0.034906), t01s = tetr1s - T(0.034906);
+    const T t02s = tetr1s + T(0.034906), t02n = tetr1n + T(0.034906);
+
+    // region: 0 loop only, 1 conical only, 2 north band, 3 south band
+    const bool loop_only = (tet0 < t01n || tet0 > t02s);
+    const bool cone_only = (tet0 > t02n && tet0 < t01s);
+    const bool north = (tet0 >= t01n && tet0 <= t02n);
+    const bool south = (tet0 >= t01s && tet0 <= t02s);
+    T px1 = x, py1 = y, pz1 = z, px2 = x, py2 = y, pz2 = z, frac = T(0.0);
+    if (north || south) {
+        const T sqr = Dm<T>::sqrt(r);
+        const T ta = north ? t01n : t01s;
+        const T tb = north ? t02n : t02s;
+        const T sa = Dm<T>::sin(ta), sb = Dm<T>::sin(tb);
+        const T st1 = sqr / Dm<T>::pow(r3 + T(1.0) / (sa * sa * sa * sa * sa * sa) - T(1.0), T(0.1666666667));
+        const T st2 = sqr / Dm<T>::pow(r3 + T(1.0) / (sb * sb * sb * sb * sb * sb) - T(1.0), T(0.1666666667));
+        T ct1 = Dm<T>::sqrt(T(1.0) - st1 * st1), ct2 = Dm<T>::sqrt(T(1.0) - st2 * st2);
+        if (south) {
+            ct1 = -ct1;
+            ct2 = -ct2;
+        }
+        T sps, cps;
+        sps = Dm<T>::sin(pas);
+        cps = Dm<T>::cos(pas);
+        px1 = r * st1 * cps;
+        py1 = r * st1 * sps;
+        pz1 = r * ct1;
+        px2 = r * st2 * cps;
+        py2 = r * st2 * sps;
+        pz2 = r * ct2;
+        const T ddx = px2 - px1, ddy = py2 - py1, ddz = pz2 - pz1;
+        const T ss = Dm<T>::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        const T qx = x - px1, qy = y - py1, qz = z - pz1;
+        const T ds = Dm<T>::sqrt(qx * qx + qy * qy + qz * qz);
+        frac = ds / ss;
+    }
+    // loop basis point: x itself, the first point (north) or the second (south)
+    const bool need_loop = loop_only || north || south;
+    const bool need_cone = cone_only || north || south;
+    v3<T> el = mk3<T>(0, 0, 0), ec = mk3<T>(0, 0, 0);
+    if (need_loop) {
+        const T lx = south ? px2 : px1, ly = south ? py2 : py1, lz = south ? pz2 : pz1;
+        el = diploop_sum(lx, ly, lz);
+        pin(el);
+    }
+    COR_SEQ();
+    if (need_cone) {
+        const T cx = north ? px2 : px1, cy = north ? py2 : py1, cz = north ? pz2 : pz1;
+        ec = condip_sum(cx, cy, cz);
+        pin(ec);
+    }
+    COR_SEQ();
+    v3<T> b;
+    if (north) {
+        b = mk3<T>(el.x * (T(1.0) - frac) + ec.x * frac, el.y * (T(1.0) - frac) + ec.y * frac,
+                   el.z * (T(1.0) - frac) + ec.z * frac);
+    } else if (south) {
+        b = mk3<T>(ec.x * (T(1.0) - frac) + el.x * frac, ec.y * (T(1.0) - frac) + el.y * frac,
+                   ec.z * (T(1.0) - frac) + el.z * frac);
+    } else if (loop_only) {
+        b = el;
+    } else if (cone_only) {
+        b = ec;
+    } else {
+        b = mk3<T>(0, 0, 0);
+    }
+    const v3<T> sh = birk1shld(x, y, z);
+    return mk3<T>(b.x + sh.x, b.y + sh.y, b.z + sh.z);
+}
+
+// lib/t96/t96.c:1196-1263
+template <class T>
+__device__ __forceinline__ v3<T> birk2shl(T x, T y, T z) {
+    const T ai[2] = {T(13.85650567), T(14.90554500)};
+    const T ak[2] = {T(10.21914434), T(10.09021632)};
+    const double amp[4] = {-111.6371348 + (double)124.5402702, 110.3735178 + (double)-122.0095905,
+                           111.9448247 + (double)-129.1957743, -110.7586562 + (double)126.5649012};
+    T szk[2], czk[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) Dm<T>::sincos(z / ak[k], &szk[k], &czk[k]);
+    v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        T sy, cy;
+        Dm<T>::sincos(y / ai[i], &sy, &cy);
+        const T aiai = T(1.0) / (ai[i] * ai[i]);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const T sq = Dm<T>::sqrt(aiai + T(1.0) / (ak[k] * ak[k]));
+            const T e = Dm<T>::exp(x * sq);
+            const T c = T(amp[2 * i + k]);
+            b.x += (-sq * e * cy * szk[k]) * c;
+            b.y += (e / ai[i] * sy * szk[k]) * c;
+            b.z += (-e / ak[k] * cy * czk[k]) * c;
+        }
+    }
+    return b;
+}
+
+// lib/t96/t96.c:1596-1627
+template <class T>
+__device__ __forceinline__ T xksi(T x, T y, T z) {
+    const T x2 = x * x, y2 = y * y, z2 = z * z;
+    const T r2 = x2 + y2 + z2;
+    const T r = Dm<T>::sqrt(r2);
+    const T xr = x / r, yr = y / r, zr = z / r;
+    const T dr = T(7.50937), dr2 = dr * dr, r0 = T(1.21563);
+    T pr = T(0.0);
+    if (r >= r0) pr = Dm<T>::sqrt((r - r0) * (r - r0) + dr2) - dr;
+    const T f = x + pr * (T(0.305662) + T(-0.383593) * xr + T(0.2677733) * xr * xr + T(-0.097656) * yr * yr + T(-0.636034) * zr * zr);
+    const T g = y + pr * (T(-0.359862) * yr + T(0.424706) * xr * yr);
+    const T h = z + pr * (T(-0.126366) * zr + T(0.292578) * xr * zr);
+    const T g2 = g * g;
+    const T fgh = f * f + g2 + h * h;
+    const T sq = Dm<T>::sqrt(fgh);
+    const T fgh32 = sq * sq * sq;
+    const T fg2 = (f * f) + g2;
+    if (fg2 < T(1.0e-5)) return T(-1.0);
+    const T ths = Dm<T>::sin(T(0.3665191) + T(0.5) * T(0.09599309) * (T(1.0) - f / Dm<T>::sqrt(fg2)));
+    return (fg2 / fgh32) - (ths * ths);
+}
+
+// lib/t96/t96.c:1629-1643
+template <class T>
+__device__ __forceinline__ T tksi(T xksi_, T xks0, T dxksi) {
+    const T tdz3 = T(2.0) * dxksi * dxksi * dxksi;
+    if (xksi_ - xks0 < -dxksi) return T(0.0);
+    if (xksi_ - xks0 >= dxksi) return T(1.0);
+    if (xksi_ >= xks0 - dxksi && xksi_ < xks0) {
+        const T u = xksi_ - xks0 + dxksi;
+        const T u3 = u * u * u;
+        return T(1.5) * u3 / (tdz3 + u3);
+    }
+    const T u = xksi_ - xks0 - dxksi;
+    const T u3 = u * u * u;
+    return T(1.0) + T(1.5) * u3 / (tdz3 - u3);
+}
+
+// lib/t96/t96.c:1414-1470 (the four loops; sin/cos of the fixed tilt angles are
+// passed in precomputed)
+template <class T>
+__device__ __forceinline__ v3<T> loops4(T x, T y, T z, T xc, T yc, T zc, T rl, T st, T ct, T sp, T cp) {
+    T xs, yss, zs, xss, zss, bxs;
+    v3<T> bl;
+    xs = (x - xc) * cp + (y - yc) * sp;
+    yss = (y - yc) * cp - (x - xc) * sp;
+    zs = z - zc;
+    xss = xs * ct - zs * st;
+    zss = zs * ct + xs * st;
+    bl = circle(xss, yss, zss, rl);
+    COR_SEQ();
+    bxs = bl.x * ct + bl.z * st;
+    const T bz1 = bl.z * ct - bl.x * st;
+    const T bx1 = bxs * cp - bl.y * sp;
+    const T by1 = bxs * sp + bl.y * cp;
+
+    xs = (x - xc) * cp - (y + yc) * sp;
+    yss = (y + yc) * cp + (x - xc) * sp;
+    zs = z - zc;
+    xss = xs * ct - zs * st;
+    zss = zs * ct + xs * st;
+    bl = circle(xss, yss, zss, rl);
+    COR_SEQ();
+    bxs = bl.x * ct + bl.z * st;
+    const T bz2 = bl.z * ct - bl.x * st;
+    const T bx2 = bxs * cp + bl.y * sp;
+    const T by2 = -bxs * sp + bl.y * cp;
+
+    xs = -(x - xc) * cp + (y + yc) * sp;
+    yss = -(y + yc) * cp - (x - xc) * sp;
+    zs = z + zc;
+    xss = xs * ct - zs * st;
+    zss = zs * ct + xs * st;
+    bl = circle(xss, yss, zss, rl);
+    COR_SEQ();
+    bxs = bl.x * ct + bl.z * st;
+    const T bz3 = bl.z * ct - bl.x * st;
+    const T bx3 = -bxs * cp - bl.y * sp;
+    const T by3 = bxs * sp - bl.y * cp;
+
+    xs = -(x - xc) * cp - (y - yc) * sp;
+    yss = -(y - yc) * cp + (x - xc) * sp;
+    zs = z + zc;
+    xss = xs * ct - zs * st;
+    zss = zs * ct + xs * st;
+    bl = circle(xss, yss, zss, rl);
+    COR_SEQ();
+    bxs = bl.x * ct + bl.z * st;
+    return mk3<T>(bx1 + bx2 + bx3 + (-bxs * cp + bl.y * sp), by1 + by2 + by3 + (-bxs * sp - bl.y * cp),
+                  bz1 + bz2 + bz3 + (bl.z * ct - bl.x * st));
+}
+
+// lib/t96/t96.c:1368-1386
+template <class T>
+__device__ __forceinline__ v3<T> dipdistr(T x, T y, T z, int mode) {
+    const T x2 = x * x;
+    const T rho2 = x2 + y * y;
+    if (mode == 0) {
+        const T r2 = rho2 + z * z;
+        const T r3 = r2 * Dm<T>::sqrt(r2);
+        return mk3<T>(z / (rho2 * rho2) * (r2 * (y * y - x2) - rho2 * x2) / r3,
+                      -x * y * z / (rho2 * rho2) * (T(2.0) * r2 + rho2) / r3, x / r3);
+    }
+    return mk3<T>(z / (rho2 * rho2) * (y * y - x2), T(-2.0) * x * y * z / (rho2 * rho2), x / rho2);
+}
+
+// lib/t96/t96.c:1323-1366
+template <class T>
+__device__ __forceinline__ v3<T> r2inner(T x, T y, T z) {
+    // bconic (lib/t96/t96.c:1338-1366)
+    const T ro2 = x * x + y * y;
+    const T ro = Dm<T>::sqrt(ro2);
+    const T cf = x / ro, sf = y / ro;
+    const T r2 = ro2 + z * z;
+    const T r = Dm<T>::sqrt(r2);
+    const T c = z / r, s = ro / r;
+    const T ch = Dm<T>::sqrt(T(0.5) * (T(1.0) + c)), sh = Dm<T>::sqrt(T(0.5) * (T(1.0) - c));
+    T tnh_prev = T(1.0), cnh_prev = T(1.0);
+    const T tnh = sh / ch, cnh = T(1.0) / tnh;
+    T cfm_prev = T(1.0), sfm_prev = T(0.0);
+    const T cc[5] = {T(154.185), T(-2.12446), T(0.601735E-01), T(-.153954E-02), T(.355077E-04)};
+    v3<T> cb[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const T cfm = cfm_prev * cf - sfm_prev * sf;
+        const T sfm = cfm_prev * sf + sfm_prev * cf;
+        cfm_prev = cfm;
+        sfm_prev = sfm;
+        const T tnhm = tnh_prev * tnh, cnhm = cnh_prev * cnh;
+        const T bt = T(m + 1) * cfm / (r * s) * (tnhm + cnhm);
+        const T bf = T(-0.5) * T(m + 1) * sfm / r * (tnh_prev / (ch * ch) - cnh_prev / (sh * sh));
+        tnh_prev = tnhm;
+        cnh_prev = cnhm;
+        cb[m] = mk3<T>(bt * c * cf - bf * sf, bt * c * sf + bf * cf, -bt * s);
+    }
+    const v3<T> d8 = loops4(x, y, z, T(-8.1902), T(6.5239), T(5.504), T(7.7815),
+                            T(0.7560782215370907), T(0.6544812624646408),
+                            T(0.04297941043769308), T(-0.9990759582126018));
+    COR_SEQ();
+    const v3<T> d6 = dipdistr(x - T(.0774), y, z, 0);
+    const v3<T> d7 = dipdistr(x + T(.038), y, z, 1);
+    return mk3<T>(cc[0] * cb[0].x + cc[1] * cb[1].x + cc[2] * cb[2].x + cc[3] * cb[3].x + cc[4] * cb[4].x + T(29.9996) * d6.x + T(262.886) * d7.x + T(99.9132) * d8.x,
+                  cc[0] * cb[0].y + cc[1] * cb[1].y + cc[2] * cb[2].y + cc[3] * cb[3].y + cc[4] * cb[4].y + T(29.9996) * d6.y + T(262.886) * d7.y + T(99.9132) * d8.y,
+                  cc[0] * cb[0].z + cc[1] * cb[1].z + cc[2] * cb[2].z + cc[3] * cb[3].z + cc[4] * cb[4].z + T(29.9996) * d6.z + T(262.886) * d7.z + T(99.9132) * d8.z);
+}
+
+// lib/t96/t96.c:1388-1412
+template <class T>
+__device__ __forceinline__ v3<T> r2outer(T x, T y, T z) {
+    const v3<T> d1 = crosslp(x, y, z, T(0.55), T(0.694), T(0.999995195003848), T(0.003099995034835719));
+    COR_SEQ();
+    const v3<T> d2 = crosslp(x, y, z, T(1.55), T(2.8), T(0.9905617592124779), T(0.13706714116038543));
+    COR_SEQ();
+    const v3<T> d3 = crosslp(x, y, z, T(-0.7), T(0.2), T(0.5714702170359919), T(0.8206228068003206));
+    COR_SEQ();
+    const v3<T> d4 = circle(x + T(2.994), y, z, T(2.925));
+    COR_SEQ();
+    const v3<T> d5 = loops4(x, y, z, T(-1.775), T(4.3), T(-0.275), T(2.7),
+                            T(0.4179612609200968), T(0.9084648503767676),
+                            T(0.999783764189357), T(0.020794827803092428));
+    return mk3<T>(T(-34.105) * d1.x + T(-2.00019) * d2.x + T(628.639) * d3.x + T(73.4847) * d4.x + T(12.5162) * d5.x,
+                  T(-34.105) * d1.y + T(-2.00019) * d2.y + T(628.639) * d3.y + T(73.4847) * d4.y + T(12.5162) * d5.y,
+                  T(-34.105) * d1.z + T(-2.00019) * d2.z + T(628.639) * d3.z + T(73.4847) * d4.z + T(12.5162) * d5.z);
+}
+
+// lib/t96/t96.c:1472-1594 (t-factors: :1474-1487)
+template <class T>
+__device__ __forceinline__ void tfac(T xks, T d1, T d2, T d3, T d34, T *t1, T *t2, T *t3) {
+    const T xks2 = xks * xks;
+    const T p72 = d2 * d2;
+    const T s7 = Dm<T>::sqrt(xks2 + p72);
+    *t1 = xks / Dm<T>::sqrt(xks2 + d1 * d1);
+    *t2 = (p72 * d2) / (s7 * s7 * s7);
+    const T q = Dm<T>::sqrt(xks2 + d3 * d3);
+    *t3 = xks / Dm<T>::powi(q, 5) * T(3.493856) * d34;
+}
+
+template <class T>
+__device__ __forceinline__ T quad(const T *c, T t1, T t2, T t3) {
+    return c[0] + c[1] * t1 + c[2] * t2 + c[3] * t3;
+}
+
+template <class T>
+__device__ __forceinline__ v3<T> r2sheet(T x, T y, T z, T xks) {
+    const T *cx = CT(k_r2s), *cy = CT(k_r2s) + 80, *cz = CT(k_r2s) + 160;
+    T t1x, t2x, t3x, t1y, t2y, t3y, t1z, t2z, t3z;
+    tfac(xks, T(0.483750e-01), T(0.396953e-01), T(0.579023e-01), T(1.124043867626711e-05), &t1x, &t2x, &t3x);
+    tfac(xks, T(0.478750e-01), T(0.363750e-01), T(0.567500e-01), T(1.0372022816406252e-05), &t1y, &t2y, &t3y);
+    tfac(xks, T(0.355625e-01), T(0.318750e-01), T(0.538750e-01), T(8.42459695336914e-06), &t1z, &t2z, &t3z);
+    const T rho2 = x * x + y * y;
+    const T r = Dm<T>::sqrt(rho2 + z * z);
+    const T rho = Dm<T>::sqrt(rho2);
+    const T c1p = x / rho, s1p = y / rho;
+    const T s2p = T(2.0) * s1p * c1p;
+    const T c2p = c1p * c1p - s1p * s1p;
+    const T s3p = s2p * c1p + c2p * s1p;
+    const T c3p = c2p * c1p - s2p * s1p;
+    const T s4p = s3p * c1p + c3p * s1p;
+    const T ct = z / r;
+    const T ct2 = ct * ct;
+    const T ct2m1 = ct2 - T(1.0);
+    // sqrt(-2e * a) of the Gaussian latitude terms (lib/t96/t96.c:1509-1511,1539-1540)
+    const T sx[5] = {T(10.189284199579348) * ct * Dm<T>::exp(T(-19.0969) * ct2),
+                     T(7.106076658978449) * ct * Dm<T>::exp(T(-9.28828) * ct2),
+                     T(0.839673526422457) * ct * Dm<T>::exp(T(-0.129687) * ct2),
+                     ct * Dm<T>::exp(T(5.58594) * ct2m1), ct * Dm<T>::exp(T(22.5055) * ct2m1)};
+    const T sy[5] = {T(8.622355131189718) * ct * Dm<T>::exp(T(-13.6750) * ct2),
+                     T(6.038125124921339) * ct * Dm<T>::exp(T(-6.70625) * ct2),
+                     ct * Dm<T>::exp(T(2.31875) * ct2m1), ct * Dm<T>::exp(T(11.4062) * ct2m1),
+                     ct * Dm<T>::exp(T(20.4562) * ct2m1)};
+    const T szz[5] = {Dm<T>::exp(T(-16.7125) * ct2), Dm<T>::exp(T(-16.4625) * ct2),
+                      Dm<T>::exp(T(-0.1625) * ct2), Dm<T>::exp(T(5.1) * (ct2 - T(1.0))),
+                      Dm<T>::exp(T(23.7125) * (ct2 - T(1.0)))};
+    v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const T *ax = cx + 16 * k, *ay = cy + 16 * k, *az = cz + 16 * k;
+        const T qx = ((quad(ax, t1x, t2x, t3x) + c1p * quad(ax + 4, t1x, t2x, t3x)) +
+                      c2p * quad(ax + 8, t1x, t2x, t3x)) + c3p * quad(ax + 12, t1x, t2x, t3x);
+        const T qy = ((s1p * quad(ay, t1y, t2y, t3y) + s2p * quad(ay + 4, t1y, t2y, t3y)) +
+                      s3p * quad(ay + 8, t1y, t2y, t3y)) + s4p * quad(ay + 12, t1y, t2y, t3y);
+        const T qz = ((quad(az, t1z, t2z, t3z) + c1p * quad(az + 4, t1z, t2z, t3z)) +
+                      c2p * quad(az + 8, t1z, t2z, t3z)) + c3p * quad(az + 12, t1z, t2z, t3z);
+        if (k == 0) {
+            b = mk3<T>(sx[0] * qx, sy[0] * qy, szz[0] * qz);
+        } else {
+            b.x = b.x + sx[k] * qx;
+            b.y = b.y + sy[k] * qy;
+            b.z = b.z + szz[k] * qz;
+        }
+    }
+    return b;
+}
+
+// lib/t96/t96.c:1265-1321.  The xi bands select outer / outer+sheet / sheet /
+// sheet+inner / inner; each representation is evaluated at most once per lane.
+template <class T>
+__device__ __forceinline__ v3<T> r2birk(T x, T y, T z) {
+    const T del = T(0.03), del1 = T(0.015);
+    const T xks = xksi(x, y, z);
+    const bool outer_only = xks < -(del + del1);
+    const bool outer_sheet = xks >= -(del + del1) && xks < -del + del1;
+    const bool sheet_only = xks >= -del + del1 && xks < del - del1;
+    const bool sheet_inner = xks >= del - del1 && xks < del + del1;
+    const bool inner_only = xks >= del + del1;
+    v3<T> bo = mk3<T>(0, 0, 0), bs = mk3<T>(0, 0, 0), bi = mk3<T>(0, 0, 0);
+    if (outer_only || outer_sheet) {
+        bo = r2outer(x, y, z);
+        pin(bo);
+    }
+    COR_SEQ();
+    if (outer_sheet || sheet_only || sheet_inner) {
+        bs = r2sheet(x, y, z, xks);
+        pin(bs);
+    }
+    COR_SEQ();
+    if (sheet_inner || inner_only) {
+        bi = r2inner(x, y, z);
+        pin(bi);
+    }
+    COR_SEQ();
+    if (outer_only) return mk3<T>(bo.x * T(-0.02), bo.y * T(-0.02), bo.z * T(-0.02));
+    if (sheet_only) return mk3<T>(bs.x * T(-0.02), bs.y * T(-0.02), bs.z * T(-0.02));
+    if (inner_only) return mk3<T>(-bi.x * T(0.02), -bi.y * T(0.02), -bi.z * T(0.02));
+    if (outer_sheet) {
+        const T f2 = T(-0.02) * tksi(xks, -del, del1);
+        const T f1 = T(-0.02) - f2;
+        return mk3<T>(bo.x * f1 + bs.x * f2, bo.y * f1 + bs.y * f2, bo.z * f1 + bs.z * f2);
+    }
+    if (sheet_inner) {
+        const T f1 = T(-0.02) * tksi(xks, del, del1);
+        const T f2 = T(-0.02) - f1;
+        return mk3<T>(bi.x * f1 + bs.x * f2, bi.y * f1 + bs.y * f2, bi.z * f1 + bs.z * f2);
+    }
+    return mk3<T>(0, 0, 0);   // xks is NaN
+}
+
+}  // namespace t96
+
+// lib/t96/t96.c:4-111: the whole model at zero tilt.
+template <class T>
+__device__ __forceinline__ v3<T> t96_field(const T96Params<T> &P, T x, T y, T z) {
+    using namespace t96;
+    const T ys = y * P.ct - z * P.st, zs = z * P.ct + y * P.st;
+    const T ysd = ys / T(10.0);
+    const T factimf = Dm<T>::exp(x / T(20.0) - ysd * ysd);
+    const T oimfy = P.oimfy0 * factimf, oimfz = P.oimfz0 * factimf;
+    const T rho2 = y * y + z * z;
+    T xmxm = P.am + x - P.c548;
+    if (xmxm < T(0.0)) xmxm = T(0.0);
+    const T axx0 = xmxm * xmxm;
+    const T aro = P.asq + rho2;
+    const T aro_axx0 = aro + axx0;
+    const T sigma = Dm<T>::sqrt((aro + axx0 + Dm<T>::sqrt(aro_axx0 * aro_axx0 - P.asq4 * axx0)) / P.asq2);
+
+    const v3<T> q = dipole(x, y, z);
+    if (sigma >= T(1.08 + 0.005)) return mk3<T>(-q.x, oimfy - q.y, oimfz - q.z);
+    const T xx = x * P.xappa, yy = y * P.xappa, zz = z * P.xappa;
+    // The sum FX = cf*xappa^3 + rcampl*rc + tampl2*t2 + tampl3*t3 + b1ampl*r1 +
+    // b2ampl*r2 + rimfampl*im (lib/t96/t96.c:91-93) is accumulated left to right as
+    // each term is produced, so no sub-model result outlives its own term.
+    v3<T> cf = cylharm(xx, yy, zz);
+    pin(cf);
+    T fx = cf.x * P.xappa3, fy = cf.y * P.xappa3, fz = cf.z * P.xappa3;
+    COR_SEQ();
+    {
+        v3<T> rc0 = ringcurr(xx, yy, zz);
+        pin(rc0);
+        COR_SEQ();
+        v3<T> w1 = shlcar3x3(0, xx, yy, zz);
+        pin(w1);
+        fx = fx + P.rcampl * (w1.x + rc0.x);
+        fy = fy + P.rcampl * (w1.y + rc0.y);
+        fz = fz + P.rcampl * (w1.z + rc0.z);
+    }
+    COR_SEQ();
+    {
+        v3<T> w2 = shlcar3x3(1, xx, yy, zz);
+        pin(w2);
+        COR_SEQ();
+        v3<T> td = taildisk(xx, yy, zz);
+        pin(td);
+        fx = fx + P.tampl2 * (w2.x + td.x);
+        fy = fy + P.tampl2 * (w2.y + td.y);
+        fz = fz + P.tampl2 * (w2.z + td.z);
+    }
+    COR_SEQ();
+    {
+        v3<T> w3 = shlcar3x3(2, xx, yy, zz);
+        pin(w3);
+        COR_SEQ();
+        T h87x, h87z;
+        tail87(xx, zz, &h87x, &h87z);
+        pin(h87x);
+        pin(h87z);
+        fx = fx + P.tampl3 * (w3.x + h87x);
+        fy = fy + P.tampl3 * w3.y;
+        fz = fz + P.tampl3 * (w3.z + h87z);
+    }
+    COR_SEQ();
+    {
+        v3<T> r1 = birk1tot(xx, yy, zz);
+        pin(r1);
+        fx = fx + P.b1ampl * r1.x;
+        fy = fy + P.b1ampl * r1.y;
+        fz = fz + P.b1ampl * r1.z;
+    }
+    COR_SEQ();
+    {
+        v3<T> r2 = birk2shl(xx, yy, zz);
+        pin(r2);
+        COR_SEQ();
+        v3<T> r2b = r2birk(xx, yy, zz);
+        pin(r2b);
+        fx = fx + P.b2ampl * (r2.x + r2b.x);
+        fy = fy + P.b2ampl * (r2.y + r2b.y);
+        fz = fz + P.b2ampl * (r2.z + r2b.z);
+    }
+    COR_SEQ();
+    {
+        // The interconnection field is a fit whose terms cancel by four orders of
+        // magnitude (coefficients ~1e7 on sin(z r_k) with r_k ~ 1e-3): evaluated in
+        // FP64 in both paths.
+        v3<T> im;
+        if (Dm<T>::fp64) {
+            im = intercon(xx, ys * P.xappa, zs * P.xappa);
+        } else {
+            const v3<double> imd = intercon<double>((double)xx, (double)(ys * P.xappa), (double)(zs * P.xappa));
+            im = mk3<T>((T)imd.x, (T)imd.y, (T)imd.z);
+        }
+        pin(im);
+        const T rimfy = im.y * P.ct + im.z * P.st;
+        const T rimfz = im.z * P.ct - im.y * P.st;
+        fx = fx + P.rimfampl * im.x;
+        fy = fy + P.rimfampl * rimfy;
+        fz = fz + P.rimfampl * rimfz;
+    }
+    if (sigma < T(1.08 - 0.005)) return mk3<T>(fx, fy, fz);
+    const T fint = T(0.5) * (T(1.0) - (sigma - T(1.08)) / T(0.005));
+    const T fext = T(0.5) * (T(1.0) + (sigma - T(1.08)) / T(0.005));
+    return mk3<T>((fx + q.x) * fint - q.x, (fy + q.y) * fint + oimfy * fext - q.y,
+                  (fz + q.z) * fint + oimfz * fext - q.z);
+}
+
+}  // namespace cor

@@ -1,0 +1,414 @@
+// tracer_dev.hpp -- the persistent cutoff tracer: one back-traced particle per lane.
+//
+// Per lane this is src/trajectory.c:12-153 (trajectory_simulation_cutoff's body) with
+// calculate_position_and_speed (:275-356) inlined.  Per wavefront it is a loop of
+//   1. refill   -- lanes whose trajectory ended take the next ids of a global queue
+//                  (one atomic per wavefront per refill, ranks by mbcnt);
+//   2. field    -- every live lane evaluates the field at its current point
+//                  (converged: all lanes are in the same phase);
+//   3. step     -- each lane repeats RK attempts until one is accepted or the
+//                  trajectory terminates (divergent, but only the cheap RK + tests).
+// A wavefront retires when the queue is drained and all its lanes have ended, so the
+// grid drains on its own.
+//
+// Precision variants:
+//   PREC_FP64  -- everything in double, the reference's operation order (compiled
+//                 without FMA contraction): the parity path.
+//   PREC_MIXED -- field models and RK stage arithmetic in FP32; position, velocity,
+//                 time and path length stay FP64 and absorb the FP32 increments;
+//                 step-size control (period, h, turn angle), the escape / re-entry
+//                 tests and the exit/asymptotic geometry stay FP64.
+#pragma once
+#include "dev_types.hpp"
+#include "igrf_dev.hpp"
+#include "t96_dev.hpp"
+
+namespace cor {
+
+// ---------------------------------------------------------------- field aggregates
+
+__device__ __forceinline__ v3<double> rot_geo_to_gsm(const double *A, v3<double> g) {
+    return mk3(A[0] * g.x + A[1] * g.y + A[2] * g.z, A[3] * g.x + A[4] * g.y + A[5] * g.z,
+               A[6] * g.x + A[7] * g.y + A[8] * g.z);
+}
+__device__ __forceinline__ v3<double> rot_gsm_to_geo(const double *A, v3<double> s) {
+    return mk3(A[0] * s.x + A[3] * s.y + A[6] * s.z, A[1] * s.x + A[4] * s.y + A[7] * s.z,
+               A[2] * s.x + A[5] * s.y + A[8] * s.z);
+}
+__device__ __forceinline__ v3<float> rot_gsm_to_geo_f(const float *A, v3<float> s) {
+    return mk3(A[0] * s.x + A[3] * s.y + A[6] * s.z, A[1] * s.x + A[4] * s.y + A[7] * s.z,
+               A[2] * s.x + A[5] * s.y + A[8] * s.z);
+}
+
+// src/models.c:194-207 with the adapters :146-192, in the reference's order: GSM [m]
+// -> Re, each model adds its GSM field (the SH model round-trips through GEO), the
+// sum is returned in GEO [nT].
+template <int INT, int EXT, int CRU>
+__device__ __forceinline__ v3<double> field_fp64(const EpochDev *__restrict__ E, v3<double> gsm_m) {
+    const v3<double> g = mk3(gsm_m.x / K_RE, gsm_m.y / K_RE, gsm_m.z / K_RE);
+    v3<double> tot = mk3(0., 0., 0.);
+    if (INT == INT_IGRF) {
+        const v3<double> geo = rot_gsm_to_geo(E->A, g);
+        const v3<double> bg = igrf_geo<double>(E->G, E->H, E->REC, geo.x, geo.y, geo.z);
+        const v3<double> bs = rot_geo_to_gsm(E->A, bg);
+        tot.x += bs.x;
+        tot.y += bs.y;
+        tot.z += bs.z;
+    }
+    if (EXT == EXT_T96) {
+        const v3<double> bs = t96_field<double>(E->t96d, g.x, g.y, g.z);
+        tot.x += bs.x;
+        tot.y += bs.y;
+        tot.z += bs.z;
+    }
+    static_assert(EXT != EXT_TS05 && CRU == CRU_NONE, "model not compiled into this tracer");
+    return rot_gsm_to_geo(E->A, tot);
+}
+
+// Mixed path: the SH model takes the GEO position directly (no GSM round trip), the
+// external model's GSM field is rotated to GEO in FP32.
+template <int INT, int EXT, int CRU>
+__device__ __forceinline__ v3<float> field_mixed(const EpochDev *__restrict__ E, v3<double> geo_m,
+                                                 v3<double> gsm_m) {
+    v3<float> tot = mk3(0.f, 0.f, 0.f);
+    if (INT == INT_IGRF) {
+        const float x = (float)(geo_m.x / K_RE), y = (float)(geo_m.y / K_RE), z = (float)(geo_m.z / K_RE);
+        tot = igrf_geo<float>(E->Gf, E->Hf, E->RECf, x, y, z);
+    }
+    if (EXT == EXT_T96) {
+        const float x = (float)(gsm_m.x / K_RE), y = (float)(gsm_m.y / K_RE), z = (float)(gsm_m.z / K_RE);
+        const v3<float> bg = rot_gsm_to_geo_f(E->Af, t96_field<float>(E->t96f, x, y, z));
+        tot.x += bg.x;
+        tot.y += bg.y;
+        tot.z += bg.z;
+    }
+    static_assert(EXT != EXT_TS05 && CRU == CRU_NONE, "model not compiled into this tracer");
+    return tot;
+}
+
+// ---------------------------------------------------------------- RK step
+
+// src/trajectory.c:275-356 in FP64: velocity stages of dv/dt = d (v x B) with B frozen,
+// position stages of the same tableau applied to f(x) = v + x; divisions kept.
+__device__ __forceinline__ void rk_step_fp64(double h, double d, v3<double> B, v3<double> &vel,
+                                             v3<double> &pos) {
+    const double vx = vel.x, vy = vel.y, vz = vel.z;
+    double kx[6], ky[6], kz[6];
+    kx[0] = h * (d * (vy * B.z - vz * B.y));
+    ky[0] = h * (d * (vz * B.x - vx * B.z));
+    kz[0] = h * (d * (vx * B.y - vy * B.x));
+#define COR_STAGE(S, CX, CY, CZ)                                   \
+    {                                                              \
+        const double ux = vx + (CX), uy = vy + (CY), uz = vz + (CZ); \
+        kx[S] = h * (d * (uy * B.z - uz * B.y));                   \
+        ky[S] = h * (d * (uz * B.x - ux * B.z));                   \
+        kz[S] = h * (d * (ux * B.y - uy * B.x));                   \
+    }
+    COR_STAGE(1, kx[0] / 3., ky[0] / 3., kz[0] / 3.)
+    COR_STAGE(2, (4. * kx[0] + 6. * kx[1]) / 25., (4. * ky[0] + 6. * ky[1]) / 25.,
+              (4. * kz[0] + 6. * kz[1]) / 25.)
+    COR_STAGE(3, (kx[0] - 12. * kx[1] + 15. * kx[2]) / 4., (ky[0] - 12. * ky[1] + 15. * ky[2]) / 4.,
+              (kz[0] - 12. * kz[1] + 15. * kz[2]) / 4.)
+    COR_STAGE(4, (6. * kx[0] + 90. * kx[1] - 50. * kx[2] + 8. * kx[3]) / 81.,
+              (6. * ky[0] + 90. * ky[1] - 50. * ky[2] + 8. * ky[3]) / 81.,
+              (6. * kz[0] + 90. * kz[1] - 50. * kz[2] + 8. * kz[3]) / 81.)
+    COR_STAGE(5, (6. * kx[0] + 36. * kx[1] + 10. * kx[2] + 8. * kx[3]) / 75.,
+              (6. * ky[0] + 36. * ky[1] + 10. * ky[2] + 8. * ky[3]) / 75.,
+              (6. * kz[0] + 36. * kz[1] + 10. * kz[2] + 8. * kz[3]) / 75.)
+#undef COR_STAGE
+    // position stages: p_s = h (v + combination of earlier p)
+    double p[3][6];
+    const double vv[3] = {vx, vy, vz};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double v0 = vv[c];
+        double *q = p[c];
+        q[0] = h * v0;
+        q[1] = h * (v0 + q[0] / 3.);
+        q[2] = h * (v0 + (4. * q[0] + 6. * q[1]) / 25.);
+        q[3] = h * (v0 + (q[0] - 12. * q[1] + 15. * q[2]) / 4.);
+        q[4] = h * (v0 + (6. * q[0] + 90. * q[1] - 50. * q[2] + 8. * q[3]) / 81.);
+        q[5] = h * (v0 + (6. * q[0] + 36. * q[1] + 10. * q[2] + 8. * q[3]) / 75.);
+    }
+    vel.x += (23. * kx[0] + 125. * kx[2] - 81. * kx[4] + 125. * kx[5]) / 192.;
+    vel.y += (23. * ky[0] + 125. * ky[2] - 81. * ky[4] + 125. * ky[5]) / 192.;
+    vel.z += (23. * kz[0] + 125. * kz[2] - 81. * kz[4] + 125. * kz[5]) / 192.;
+    pos.x += (23. * p[0][0] + 125. * p[0][2] - 81. * p[0][4] + 125. * p[0][5]) / 192.;
+    pos.y += (23. * p[1][0] + 125. * p[1][2] - 81. * p[1][4] + 125. * p[1][5]) / 192.;
+    pos.z += (23. * p[2][0] + 125. * p[2][2] - 81. * p[2][4] + 125. * p[2][5]) / 192.;
+}
+
+// Same tableau with the stage arithmetic in FP32: only the increments dv, dx are formed
+// (in FP32, relative to the FP64 velocity), and they are added to the FP64 state.
+// Tableau divisions become multiplications by the FP32 reciprocals.
+__device__ __forceinline__ void rk_step_mixed(double h_d, double d_d, v3<float> B, v3<double> &vel,
+                                              v3<double> &pos) {
+    const float h = (float)h_d, hd = (float)(h_d * d_d);
+    const float vx = (float)vel.x, vy = (float)vel.y, vz = (float)vel.z;
+    float kx[6], ky[6], kz[6];
+    kx[0] = hd * (vy * B.z - vz * B.y);
+    ky[0] = hd * (vz * B.x - vx * B.z);
+    kz[0] = hd * (vx * B.y - vy * B.x);
+#define COR_STAGE(S, CX, CY, CZ)                                  \
+    {                                                             \
+        const float ux = vx + (CX), uy = vy + (CY), uz = vz + (CZ); \
+        kx[S] = hd * (uy * B.z - uz * B.y);                       \
+        ky[S] = hd * (uz * B.x - ux * B.z);                       \
+        kz[S] = hd * (ux * B.y - uy * B.x);                       \
+    }
+    const float r3 = 1.f / 3.f, r25 = 1.f / 25.f, r4 = 0.25f, r81 = 1.f / 81.f, r75 = 1.f / 75.f;
+    COR_STAGE(1, kx[0] * r3, ky[0] * r3, kz[0] * r3)
+    COR_STAGE(2, (4.f * kx[0] + 6.f * kx[1]) * r25, (4.f * ky[0] + 6.f * ky[1]) * r25,
+              (4.f * kz[0] + 6.f * kz[1]) * r25)
+    COR_STAGE(3, (kx[0] - 12.f * kx[1] + 15.f * kx[2]) * r4, (ky[0] - 12.f * ky[1] + 15.f * ky[2]) * r4,
+              (kz[0] - 12.f * kz[1] + 15.f * kz[2]) * r4)
+    COR_STAGE(4, (6.f * kx[0] + 90.f * kx[1] - 50.f * kx[2] + 8.f * kx[3]) * r81,
+              (6.f * ky[0] + 90.f * ky[1] - 50.f * ky[2] + 8.f * ky[3]) * r81,
+              (6.f * kz[0] + 90.f * kz[1] - 50.f * kz[2] + 8.f * kz[3]) * r81)
+    COR_STAGE(5, (6.f * kx[0] + 36.f * kx[1] + 10.f * kx[2] + 8.f * kx[3]) * r75,
+              (6.f * ky[0] + 36.f * ky[1] + 10.f * ky[2] + 8.f * ky[3]) * r75,
+              (6.f * kz[0] + 36.f * kz[1] + 10.f * kz[2] + 8.f * kz[3]) * r75)
+#undef COR_STAGE
+    const float r192 = 1.f / 192.f;
+    float dp[3];
+    const float vv[3] = {vx, vy, vz};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float v0 = vv[c];
+        const float q0 = h * v0;
+        const float q1 = h * (v0 + q0 * r3);
+        const float q2 = h * (v0 + (4.f * q0 + 6.f * q1) * r25);
+        const float q3 = h * (v0 + (q0 - 12.f * q1 + 15.f * q2) * r4);
+        const float q4 = h * (v0 + (6.f * q0 + 90.f * q1 - 50.f * q2 + 8.f * q3) * r81);
+        const float q5 = h * (v0 + (6.f * q0 + 36.f * q1 + 10.f * q2 + 8.f * q3) * r75);
+        dp[c] = (23.f * q0 + 125.f * q2 - 81.f * q4 + 125.f * q5) * r192;
+    }
+    vel.x += (double)((23.f * kx[0] + 125.f * kx[2] - 81.f * kx[4] + 125.f * kx[5]) * r192);
+    vel.y += (double)((23.f * ky[0] + 125.f * ky[2] - 81.f * ky[4] + 125.f * ky[5]) * r192);
+    vel.z += (double)((23.f * kz[0] + 125.f * kz[2] - 81.f * kz[4] + 125.f * kz[5]) * r192);
+    pos.x += (double)dp[0];
+    pos.y += (double)dp[1];
+    pos.z += (double)dp[2];
+}
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double norm3(v3<double> v) {
+    return sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+}
+
+// global id -> scan (scans are sorted by their first global id)
+__device__ __forceinline__ int find_scan(const ScanDev *__restrict__ scans, int n, int64_t gid) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (scans[mid].first <= gid) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int PREC>
+struct FieldT;
+template <>
+struct FieldT<PREC_FP64> { typedef double T; };
+template <>
+struct FieldT<PREC_MIXED> { typedef float T; };
+
+// ---------------------------------------------------------------- the kernel
+
+// Register budget: COR_WAVES_PER_EU wavefronts per SIMD (set per translation unit; the
+// FP64 T96 tracer needs ~250 VGPRs, so 2; the FP32 field allows more).
+#ifndef COR_WAVES_PER_EU
+#define COR_WAVES_PER_EU 2
+#endif
+
+template <int PREC, int INT, int EXT, int CRU>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COR_WAVES_PER_EU)))
+trace_kernel(TraceArgs a) {
+    typedef typename FieldT<PREC>::T FT;
+    const EpochDev *__restrict__ E = a.epoch;
+    const int lane = __lane_id();
+
+    bool active = false;
+    bool queue_open = true;
+    int64_t gid = 0;
+    int scan = 0;
+    v3<double> pos = mk3(0., 0., 0.), vel = mk3(0., 0., 0.);
+    v3<FT> fld = mk3<FT>(0, 0, 0);
+    double v = 0., b = 0., time = 0., alength = 0., zn = 0.;
+    int nk0 = 0, nk1 = 0;
+    unsigned int step = 0;
+    uint32_t attempts = 0;
+
+    for (;;) {
+        // ---- 1. refill ended lanes from the queue
+        if (queue_open) {
+            const uint64_t idle = __ballot(!active);
+            if (idle) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(a.head, (unsigned long long)__popcll(idle));
+                base = __shfl(base, leader);
+                if (base + __popcll(idle) >= (unsigned long long)a.n_items) queue_open = false;
+                if (!active) {
+                    const unsigned long long slot = base + __popcll(idle & ((1ull << lane) - 1ull));
+                    if (slot < (unsigned long long)a.n_items) {
+                        gid = a.order ? (int64_t)a.order[slot] + a.id_base : (int64_t)slot + a.id_base;
+                        scan = find_scan(a.scans, a.n_scans, gid);
+                        const ScanDev &S = a.scans[scan];
+                        const int i = (int)(gid - S.first);
+                        // src/trajectory.c:13-43
+                        zn = S.zn;
+                        nk1 = S.nk1;
+                        nk0 = nk1;
+                        const double rig = S.rig + i * S.del;
+                        const double aa = rig * 1.0e+09 * zn * K_Q / (a.mass * K_C * K_C);
+                        v = K_C * sqrt(1. - 1. / (1. + aa * aa));
+                        const double rs = v * S.st1;
+                        vel = mk3(rs * S.cp1, rs * S.sp1, v * S.ct1);
+                        pos = mk3(S.pos0[0], S.pos0[1], S.pos0[2]);
+                        time = 0.;
+                        alength = 0.;
+                        step = 0;
+                        attempts = 0;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) break;
+
+        if (active) {
+            // ---- 2. field at the current point (src/trajectory.c:37-38,147-148); the
+            // GSM position is recomputed from the GEO state (same arithmetic as the
+            // reference's geo_to_gsm after an accepted step)
+            const v3<double> gsm = rot_geo_to_gsm(E->A, pos);
+            if (PREC == PREC_FP64) {
+                const v3<double> f = field_fp64<INT, EXT, CRU>(E, gsm);
+                fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
+                b = 1.E-09 * sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
+            } else {
+                fld = mk3<FT>(0, 0, 0);
+                const v3<float> f = field_mixed<INT, EXT, CRU>(E, pos, gsm);
+                fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
+                const double fx = f.x, fy = f.y, fz = f.z;
+                b = 1.E-09 * sqrt(fx * fx + fy * fy + fz * fz);
+            }
+
+            // ---- 3. attempts until one is accepted or the trajectory ends
+            int status = -1;
+            for (;;) {
+                step++;
+                const double vc = v / K_C;
+                if (vc > 1.) {
+                    status = ST_FORBIDDEN_REENTRY;
+                    break;
+                }
+                const double hm = a.mass / sqrt(1. - vc * vc);
+                const double period = (2. * K_PI * hm) / (fabs(zn) * K_Q * b);
+                const double h = period / nk0;
+                const double d = (1.E-09 * zn * K_Q) / hm;
+                const v3<double> pos0 = pos, vel0 = vel;
+                const double vprev = v;
+                if (PREC == PREC_FP64) {
+                    rk_step_fp64(h, d, mk3<double>(fld.x, fld.y, fld.z), vel, pos);
+                } else {
+                    rk_step_mixed(h, d, mk3<float>(fld.x, fld.y, fld.z), vel, pos);
+                }
+                attempts++;
+                v = norm3(vel);
+                if (__builtin_isnan(v)) {
+                    // a NaN state never meets a termination test: the reference runs it
+                    // to the step limit and classifies it as forbidden there
+                    step = (unsigned int)(a.step_limit + 1);
+                    status = ST_FORBIDDEN_LIMIT;
+                    break;
+                }
+                const double gc = (vel.x * vel0.x + vel.y * vel0.y + vel.z * vel0.z) / (v * vprev);
+                bool reject;
+                if (PREC == PREC_FP64) {
+                    reject = false;
+                    if (gc < 1.) {
+                        const double gu = atan2(sqrt((1. - gc) * (1. + gc)), gc);
+                        reject = gu >= a.tu_angle;
+                    }
+                } else {
+                    reject = gc <= a.cos_tu;   // acos(gc) >= tu_angle
+                }
+                if (reject && nk0 <= 50000) {
+                    nk0 = (int)((unsigned)nk0 * 2u);   // wraps like the reference's int
+                    pos = pos0;
+                    vel = vel0;
+                    step--;
+                    continue;
+                }
+                const v3<double> gsm = rot_geo_to_gsm(E->A, pos);
+                const double pax = (gsm.x / K_RE + 25.3) / 36.08;
+                const double paz = (gsm.y * gsm.y + gsm.z * gsm.z) / (K_RE * K_RE);
+                const double r = norm3(pos) / K_RE;
+                const double paus = pax * pax + paz / 459.6736;
+                if (r >= 25. || paus > 1.) {
+                    if ((r >= 25. && fabs(r - 25.) < 0.002) ||
+                        (r <= 25. && paus > 1. && fabs(paus - 1.) < 0.002)) {
+                        status = ST_ALLOWED;
+                        // src/trajectory.c:81,91-112: exit point and asymptotic direction
+                        const double rho2 = pos.x * pos.x + pos.y * pos.y;
+                        double th = 0., ph = 0.;
+                        if (rho2 != 0.) {
+                            ph = atan2(pos.y, pos.x);
+                            th = atan2(sqrt(rho2), pos.z);
+                            if (ph < 0.) ph += K_2PI;
+                        } else if (pos.z < 0.) {
+                            th = K_PI;
+                        }
+                        const double st = sin(th), ct = cos(th), sf = sin(ph), cf = cos(ph);
+                        const double be = vel.x * cf + vel.y * sf;
+                        const double vr = be * st + vel.z * ct;
+                        const double vth = be * ct - vel.z * st;
+                        const double vph = -vel.x * sf + vel.y * cf;
+                        const double va = vth * ct + vr * st;
+                        const double dm = sqrt(vth * vth + va * va);
+                        double asf = (ph + atan2(vph, va)) / K_TO_RAD;
+                        if (asf > 360.) asf -= 360.;
+                        a.out.vc[gid] = v / K_C;
+                        a.out.r[gid] = r;
+                        a.out.td[gid] = 90. - (th / K_TO_RAD);
+                        a.out.fei[gid] = ph / K_TO_RAD;
+                        a.out.ast[gid] = atan2(-vth * st + vr * ct, dm) / K_TO_RAD;
+                        a.out.asf[gid] = asf;
+                        a.out.time[gid] = time;
+                        a.out.length[gid] = alength;
+                        break;
+                    }
+                    nk0 = (int)((unsigned)nk0 * 2u);   // wraps like the reference's int
+                    pos = pos0;
+                    vel = vel0;
+                    step--;
+                    continue;
+                }
+                if (r < 1.) {
+                    status = ST_FORBIDDEN_REENTRY;
+                    break;
+                }
+                if ((uint64_t)step > a.step_limit) {
+                    status = ST_FORBIDDEN_LIMIT;
+                    break;
+                }
+                // accepted, continuing: src/trajectory.c:150-152
+                time = time + period / nk0;
+                const double dx = pos.x - pos0.x, dy = pos.y - pos0.y, dz = pos.z - pos0.z;
+                alength += sqrt(dx * dx + dy * dy + dz * dz);
+                nk0 = nk1;
+                break;
+            }
+            if (status >= 0) {
+                a.out.status[gid] = status;
+                a.out.steps[gid] = step;
+                a.out.attempts[gid] = attempts;
+                active = false;
+            }
+        }
+    }
+}
+
+}  // namespace cor

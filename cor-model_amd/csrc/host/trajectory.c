@@ -1,0 +1,41 @@
+/*
+ * trajectory.c -- trajectory_simulation_cutoff of the reference API
+ * (src/trajectory.c:6-161): the whole rigidity scan is traced by the GPU tracer in
+ * FP64 and the result is printed exactly as the reference prints it (one line per
+ * allowed rigidity in ascending order, then the cutoff summary).
+ */
+#include <stdlib.h>
+#include "cor_host.h"
+
+void trajectory_simulation_cutoff(InfileHeader *header, const Model **models, ModelsParams *models_params,
+                                  FILE *outfile, uint64_t step_limit, double tu_angle,
+                                  double particle_mass, uint32_t parallel) {
+    (void)parallel;
+    cor_model_set set;
+    const cor_scan scan = {header->r0, header->the0, header->fi0, header->the1, header->fi1,
+                           header->rig, header->del, header->rk, header->zn, header->nk1, 0};
+    const cor_trace_opts opts = {step_limit, tu_angle, particle_mass, COR_PREC_FP64, 0};
+    int rc = cor_models_to_set(models, &set);
+    const int64_t n = cor_scan_count(&scan);
+    if (rc == COR_OK && n < 0) rc = COR_E_INVALID;
+    cor_traj *traj = NULL;
+    cor_cutoff cut;
+    if (rc == COR_OK) {
+        traj = (cor_traj *)calloc((size_t)(n > 0 ? n : 1), sizeof(cor_traj));
+        rc = traj ? cor_gpu_trace(&set, models_params, &scan, 1, &opts, traj, &cut) : COR_E_NOMEM;
+    }
+    if (rc != COR_OK) {
+        fprintf(stderr, "Error: GPU tracer failed: %s (%s)\n", cor_gpu_error_string(rc), cor_gpu_last_hip_error());
+        free(traj);
+        return;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const cor_traj *t = &traj[i];
+        if (t->status != COR_ALLOWED) continue;
+        const double rig = header->rig + (int)i * header->del;   /* src/trajectory.c:14 */
+        fprintf(outfile, "%10.6f%15.10f%12.6f%10.3f%10.3f%10.3f%10.3f%12.6f%16.2f\n", rig, t->vc, t->r, t->td,
+                t->fei, t->ast, t->asf, t->time, t->length_m / 1000.);
+    }
+    fprintf(outfile, "  CUTOFF with rigidities P(S),P(C),P(M) are:\n%12.5f%12.5f%12.5f\n\n", cut.rl, cut.ru, cut.rc);
+    free(traj);
+}

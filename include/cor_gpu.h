@@ -1,0 +1,154 @@
+/*
+ * cor_gpu.h -- batch C-ABI of the MI355X cutoff-rigidity tracer (gfx950).
+ *
+ * Plain pointers and sizes only.  Host buffers are caller-owned; the library owns its
+ * device buffers.  Every call returns 0 or a negative COR_E* code
+ * (cor_gpu_error_string).  Calls on one plan are single-threaded.
+ *
+ * Counterparts in the reference:
+ *   cor_gpu_trace / cor_trace_plan_*  replace the per-rigidity loop of
+ *       trajectory_simulation_cutoff (src/trajectory.c:6-161, declared
+ *       src/trajectory.h:10), batched over many (station, direction, scan) items;
+ *   cor_gpu_field  replaces per-point calculate_field (src/models.c:194-207,
+ *       declared src/models.h:32) over a batch of points;
+ *   cor_cutoff  is the summary line of src/trajectory.c:156-160;
+ *   cor_traj  carries the per-rigidity line of src/trajectory.c:118-122.
+ */
+#ifndef COR_GPU_H
+#define COR_GPU_H
+
+#include <stdint.h>
+#include "cor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    COR_OK = 0,
+    COR_E_INVALID = -1,       /* bad argument (sizes, pointers, model set, scan) */
+    COR_E_NO_DEVICE = -2,     /* no GPU, or the library was not built for it */
+    COR_E_HIP = -3,           /* a HIP runtime call failed (cor_gpu_last_hip_error) */
+    COR_E_UNSUPPORTED = -4,   /* model combination not compiled into the tracer */
+    COR_E_NOMEM = -5
+};
+
+enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1 };
+enum { COR_INTERNAL_NONE = 0, COR_INTERNAL_IGRF = 1 };
+enum { COR_EXTERNAL_NONE = 0, COR_EXTERNAL_T96 = 1, COR_EXTERNAL_TS05 = 2 };
+enum { COR_CRUSTAL_NONE = 0, COR_CRUSTAL_SH = 1 };
+enum { COR_ALLOWED = 0, COR_FORBIDDEN_REENTRY = 1, COR_FORBIDDEN_LIMIT = 2 };
+
+#define COR_SH_NM_MAX 24   /* largest SH table (rows = cols) the device accepts */
+
+/* Which evaluators the field sums, in the reference's order (internal, external,
+ * crustal; src/main.c:268-299).  The SH table is [sh_nm_max][sh_nm_max], row n,
+ * column m, nT, as lib/chaos/chaos.c:198-199 indexes it. */
+typedef struct {
+    int32_t internal;
+    int32_t external;
+    int32_t crustal;
+    int32_t sh_nm_max;
+    const double *sh_g;
+    const double *sh_h;
+} cor_model_set;
+
+/* One station x one arrival direction x one rigidity scan: the content of one
+ * reference input header (src/io.c:4-20, src/common.h:26-54).  The scan is
+ * rig + i*del for i = 0 .. round((rk - rig)/del). */
+typedef struct {
+    double r0, the0, fi0;     /* station radius [Re], geocentric latitude, longitude [deg] */
+    double the1, fi1;         /* arrival direction as GEO latitude, longitude [deg] */
+    double rig, del, rk;      /* first rigidity, step, last rigidity [GV] */
+    double zn;                /* charge number; -1 back-traces a proton */
+    int32_t nk1;              /* steps per gyro-period each step starts from */
+    int32_t pad;
+} cor_scan;
+
+/* Tracer options: the reference's cutoff() arguments (src/main.c:12-30). */
+typedef struct {
+    uint64_t step_limit;      /* -s (reference default 25000) */
+    double tu_angle;          /* -u [rad] (default 0.01) */
+    double particle_mass;     /* DEF_HM0_P or DEF_HM0_E */
+    int32_t precision;        /* COR_PREC_FP64 or COR_PREC_MIXED */
+    int32_t queue_order;      /* 0 = rigidity order, 1 = longest-expected first */
+} cor_trace_opts;
+
+/* Per trajectory.  The doubles are set only for allowed trajectories: exit v/c, exit
+ * radius [Re], exit latitude/longitude, asymptotic latitude/longitude [deg], flight
+ * time [s] and path length [m]. */
+typedef struct {
+    int32_t status;           /* COR_ALLOWED / COR_FORBIDDEN_* */
+    uint32_t steps;           /* accepted integration steps */
+    uint32_t attempts;        /* RK attempts including rejected ones */
+    uint32_t pad;
+    double vc, r, td, fei, ast, asf, time, length_m;
+} cor_traj;
+
+/* Per scan: the reference's P(S), P(C), P(M) = Rl, Ru, Rc (src/trajectory.c:156-160). */
+typedef struct {
+    double rl, ru, rc;
+    int32_t n_forbidden;
+    int32_t n;
+} cor_cutoff;
+
+typedef struct {
+    double kernel_ms;         /* tracer kernel time of the last run (HIP events) */
+    double total_ms;          /* queue reset + tracer + reduction of the last run */
+    uint64_t steps;           /* accepted steps of the last run */
+    uint64_t attempts;        /* RK attempts of the last run */
+    int64_t n_traj;
+    int32_t grid_blocks;
+    int32_t block_threads;
+} cor_trace_stats;
+
+typedef struct cor_trace_plan cor_trace_plan;
+
+/* Device to run on (default 0).  Returns COR_E_NO_DEVICE when there is none. */
+int cor_gpu_init(int device);
+const char *cor_gpu_error_string(int code);
+const char *cor_gpu_last_hip_error(void);
+/* Wait for all work the library queued on the device. */
+int cor_gpu_synchronize(void);
+
+/* Upload one epoch's parameters and a list of scans; allocate the result arrays. */
+int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch,
+                          const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
+                          cor_trace_plan **plan);
+/* Trace every trajectory of the plan and reduce the cutoffs, on `stream`
+ * (a hipStream_t; NULL = the library's own stream).  Asynchronous. */
+int cor_trace_plan_run(cor_trace_plan *plan, void *stream);
+int cor_trace_plan_sync(cor_trace_plan *plan);
+int64_t cor_trace_plan_count(const cor_trace_plan *plan);
+/* Copy results to the host (either pointer may be NULL).  traj has
+ * cor_trace_plan_count() entries in scan order; cut has n_scans. */
+int cor_trace_plan_results(cor_trace_plan *plan, cor_traj *traj, cor_cutoff *cut);
+int cor_trace_plan_stats(cor_trace_plan *plan, cor_trace_stats *stats);
+void cor_trace_plan_destroy(cor_trace_plan *plan);
+
+/* create + run + results + destroy */
+int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                  int32_t n_scans, const cor_trace_opts *opts, cor_traj *traj,
+                  cor_cutoff *cut);
+
+/* calculate_field over n points: GSM position [m] (n x 3) -> GEO field [nT] (n x 3). */
+int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
+                  double *geo_nT, int64_t n, int32_t precision);
+
+/* round((rk - rig)/del) + 1 as src/trajectory.c:7 computes it, or COR_E_INVALID. */
+int64_t cor_scan_count(const cor_scan *scan);
+
+/* Site-list helpers (FP64, host).  No counterpart in the reference, which takes a
+ * geocentric station and a GEO-angle direction; they convert WGS84 geodetic
+ * latitude/height [deg, km] to the reference's r0 [Re = 6371.2 km] and geocentric
+ * latitude, and a local zenith/azimuth [deg, azimuth clockwise from north] to the
+ * reference's (the1, fi1). */
+int cor_geodetic_to_geocentric(double lat_deg, double h_km, double *r0_re, double *lat_c_deg);
+int cor_direction_to_geo(double lat_deg, double lon_deg, double zenith_deg, double azimuth_deg,
+                         double *the1_deg, double *fi1_deg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COR_GPU_H */

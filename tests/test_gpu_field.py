@@ -1,0 +1,79 @@
+"""GPU field evaluators against the oracle (calculate_field, src/models.c:194-207).
+
+FP64 path: the same expression order as the reference, compiled without FMA
+contraction; the remaining difference is OCML vs glibc transcendentals (a few ULP),
+so the bar is 1e-12 relative to |B|.
+
+Mixed path: FP32 field.  T96's shielding fits cancel terms 1e4-1e5 times larger than
+their sum, so its FP32 error is measured against the field the particle sees: |dB|
+divided by |B| + B_dip(r) (B_dip = 30574 nT / r^3, the dipole scale, which is what
+IGRF adds on top of the external field).  Bar: max 1e-3, 99th percentile 1e-4.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as L
+
+pytestmark = pytest.mark.gpu
+
+RE = 6.3712e6
+
+
+def points(seed, n, rmin=1.0, rmax=25.0):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = rng.uniform(rmin, rmax, size=(n, 1))
+    return d * r * RE
+
+
+def box_points(seed, n):
+    """GSM points spread over the T96 domain, tail included (x down to -40 Re)."""
+    rng = np.random.default_rng(seed)
+    p = np.column_stack([rng.uniform(-40, 12, n), rng.uniform(-18, 18, n), rng.uniform(-18, 18, n)])
+    p = p[np.linalg.norm(p, axis=1) > 1.0]
+    return p * RE
+
+
+CASES = [
+    ("igrf", dict(internal=1, external=0), None, points(1, 4000)),
+    ("t96", dict(internal=0, external=1), [2, -20, 0, -2], box_points(2, 6000)),
+    ("igrf+t96", dict(internal=1, external=1), [2, -20, 0, -2], box_points(3, 6000)),
+    ("igrf+t96 storm", dict(internal=1, external=1), [6, -150, 4, -8], box_points(4, 4000)),
+    ("igrf+t96 quiet", dict(internal=1, external=1), [0.8, 5, -3, 2], box_points(5, 4000)),
+]
+
+
+@pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
+def test_field_fp64_matches_oracle(cor, oracle, name, kinds, parmod, pts):
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[parmod[0], parmod[1], parmod[2], parmod[3]] if parmod else None)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    if parmod:
+        for k, v in enumerate(parmod):
+            po.PARMOD[k] = v
+    got = cor.field(cor.model_set(**kinds), p, pts, cor.PREC_FP64)
+    want = oracle.field(L.models(**kinds), po, pts)
+    scale = np.linalg.norm(want, axis=1)
+    err = np.linalg.norm(got - want, axis=1) / scale
+    print(name, "max rel err", err.max())
+    assert np.all(np.isfinite(got))
+    assert err.max() < 1e-12
+
+
+@pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
+def test_field_mixed_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    if parmod:
+        for k, v in enumerate(parmod):
+            po.PARMOD[k] = v
+    got = cor.field(cor.model_set(**kinds), p, pts, cor.PREC_MIXED)
+    want = oracle.field(L.models(**kinds), po, pts)
+    r = np.linalg.norm(pts, axis=1) / RE
+    scale = np.linalg.norm(want, axis=1) + 30574.0 / r ** 3
+    err = np.linalg.norm(got - want, axis=1) / scale
+    print(name, "mixed max rel err", err.max(), "p99", np.quantile(err, 0.99),
+          "max abs nT", np.linalg.norm(got - want, axis=1).max())
+    assert np.all(np.isfinite(got))
+    assert err.max() < 1e-3
+    assert np.quantile(err, 0.99) < 1e-4
