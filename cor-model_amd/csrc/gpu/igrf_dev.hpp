@@ -4,17 +4,23 @@
 // are carried up in degree with the REC table, cos/sin(m phi) are advanced by rotation,
 // and the expansion stops at the radius-dependent degree NM = min(13, 3 + 30/((int)R+2)).
 //
-// GPU layout: the (m, n) loops run to the wavefront's largest truncation (found with
-// ballots, so only active lanes count) with wave-uniform trip counts.  The coefficient
-// index n(n-1)/2 + m - 1 is therefore uniform and G/H/REC are read with scalar loads
-// from the epoch block (constant for the whole launch) -- every lane of the wavefront
-// consumes the same coefficient in the same cycle; lanes whose own truncation is lower
-// mask their accumulation.  The (1/R)^(n+1) powers are produced by the same
-// left-to-right product chain as the reference's A[] table, without a register array.
+// GPU layout: G/H/REC of the epoch are staged in LDS once per workgroup (the kernels
+// are persistent, so this is a one-time copy); the (m, n) loops run to the
+// wavefront's largest truncation (found with ballots, so only active lanes count)
+// with wave-uniform trip counts, so every lane reads the same coefficient in the same
+// cycle (an LDS broadcast) and lanes whose own truncation is lower mask their
+// accumulation.  The next degree's coefficients are read one iteration ahead so the
+// LDS latency overlaps the recursion arithmetic.  The (1/R)^(n+1) powers are
+// produced by the same left-to-right product chain as the reference's A[] table,
+// without a register array.
 #pragma once
 #include "devmath.hpp"
 
 namespace cor {
+
+// Coefficient tables as staged in LDS: 105 entries padded so the one-ahead prefetch of
+// the last degree stays inside the array.
+constexpr int IGRF_LDS_N = 128;
 
 template <class T>
 __device__ __forceinline__ v3<T> igrf_geo(const T *__restrict__ G, const T *__restrict__ H,
@@ -50,9 +56,11 @@ __device__ __forceinline__ v3<T> igrf_geo(const T *__restrict__ G, const T *__re
         T q = pmm, dq = dmm, q2 = T(0.), dq2 = T(0.), bphi = T(0.);
         T an = am;
         int idx = m * (m - 1) / 2 + m - 1;
+        T g = G[idx], h = H[idx], rec = REC[idx];
         for (int n = m; n <= kw; n++) {
+            // prefetch degree n+1 (the LDS tables are padded, so idx + n stays in range)
+            const T gn = G[idx + n], hn = H[idx + n], recn = REC[idx + n];
             const bool on = n <= klane;
-            const T g = G[idx], h = H[idx], rec = REC[idx];
             const T w = g * sm + h * cm;
             const T bn = an * n;
             if (on) {
@@ -71,6 +79,9 @@ __device__ __forceinline__ v3<T> igrf_geo(const T *__restrict__ G, const T *__re
             q = pnext;
             an = an * inv_r;
             idx += n;   // n(n+1)/2 + m - 1 for the next degree
+            g = gn;
+            h = hn;
+            rec = recn;
         }
         dmm = s * dmm + c * pmm;
         pmm = s * pmm;

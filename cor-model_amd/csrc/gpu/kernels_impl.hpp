@@ -9,15 +9,18 @@ namespace cor {
 // calculate_field over a batch of points (src/models.c:194-207), one point per lane.
 template <int PREC, int INT, int EXT, int CRU>
 __global__ void __launch_bounds__(256) field_batch_kernel(FieldBatchArgs a) {
+    typedef typename FieldT<PREC>::T FT;
+    __shared__ FT s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
+    const EpochView<FT> E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const v3<double> gsm = mk3(a.gsm_m[3 * i], a.gsm_m[3 * i + 1], a.gsm_m[3 * i + 2]);
     v3<double> out;
-    if (PREC == PREC_FP64) {
-        out = field_fp64<INT, EXT, CRU>(a.epoch, gsm);
+    if constexpr (PREC == PREC_FP64) {
+        out = field_fp64<INT, EXT, CRU>(E, gsm);
     } else {
-        const v3<double> geo = rot_gsm_to_geo(a.epoch->A, gsm);
-        const v3<float> f = field_mixed<INT, EXT, CRU>(a.epoch, geo, gsm);
+        const v3<double> geo = rot_gsm_to_geo(E.A, gsm);
+        const v3<float> f = field_mixed<INT, EXT, CRU>(E, geo, gsm);
         out = mk3((double)f.x, (double)f.y, (double)f.z);
     }
     a.geo_nT[3 * i] = out.x;

@@ -25,6 +25,52 @@
 
 namespace cor {
 
+// ---------------------------------------------------------------- epoch constants
+
+// What the field evaluators read per step: the IGRF tables from LDS, the rotation rows
+// and the T96 constants from registers.  Built once per workgroup before the
+// persistent loop (stage_epoch), so the step does no global-memory reads.
+template <class FT>
+struct EpochView {
+    const FT *G, *H, *REC;
+    double A[9];
+    float Af[9];
+    T96Params<FT> t96;
+};
+
+__device__ __forceinline__ const T96Params<double> &t96_of(const EpochDev *E, double) { return E->t96d; }
+__device__ __forceinline__ const T96Params<float> &t96_of(const EpochDev *E, float) { return E->t96f; }
+__device__ __forceinline__ const double *igrf_tab(const EpochDev *E, int k, double) {
+    return k == 0 ? E->G : (k == 1 ? E->H : E->REC);
+}
+__device__ __forceinline__ const float *igrf_tab(const EpochDev *E, int k, float) {
+    return k == 0 ? E->Gf : (k == 1 ? E->Hf : E->RECf);
+}
+
+// Copy the epoch's IGRF tables into the workgroup's LDS (zero padding) and the small
+// constants into the view.  Called by every thread of the workgroup.
+template <class FT>
+__device__ __forceinline__ EpochView<FT> stage_epoch(const EpochDev *__restrict__ E, FT *sg, FT *sh, FT *srec) {
+    const FT *g = igrf_tab(E, 0, FT(0)), *h = igrf_tab(E, 1, FT(0)), *rc = igrf_tab(E, 2, FT(0));
+    for (int k = threadIdx.x; k < IGRF_LDS_N; k += blockDim.x) {
+        sg[k] = k < NCOEF ? g[k] : FT(0);
+        sh[k] = k < NCOEF ? h[k] : FT(0);
+        srec[k] = k < NCOEF ? rc[k] : FT(0);
+    }
+    __syncthreads();
+    EpochView<FT> V;
+    V.G = sg;
+    V.H = sh;
+    V.REC = srec;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        V.A[k] = E->A[k];
+        V.Af[k] = E->Af[k];
+    }
+    V.t96 = t96_of(E, FT(0));
+    return V;
+}
+
 // ---------------------------------------------------------------- field aggregates
 
 __device__ __forceinline__ v3<double> rot_geo_to_gsm(const double *A, v3<double> g) {
@@ -44,40 +90,40 @@ __device__ __forceinline__ v3<float> rot_gsm_to_geo_f(const float *A, v3<float> 
 // -> Re, each model adds its GSM field (the SH model round-trips through GEO), the
 // sum is returned in GEO [nT].
 template <int INT, int EXT, int CRU>
-__device__ __forceinline__ v3<double> field_fp64(const EpochDev *__restrict__ E, v3<double> gsm_m) {
+__device__ __forceinline__ v3<double> field_fp64(const EpochView<double> &E, v3<double> gsm_m) {
     const v3<double> g = mk3(gsm_m.x / K_RE, gsm_m.y / K_RE, gsm_m.z / K_RE);
     v3<double> tot = mk3(0., 0., 0.);
     if (INT == INT_IGRF) {
-        const v3<double> geo = rot_gsm_to_geo(E->A, g);
-        const v3<double> bg = igrf_geo<double>(E->G, E->H, E->REC, geo.x, geo.y, geo.z);
-        const v3<double> bs = rot_geo_to_gsm(E->A, bg);
+        const v3<double> geo = rot_gsm_to_geo(E.A, g);
+        const v3<double> bg = igrf_geo<double>(E.G, E.H, E.REC, geo.x, geo.y, geo.z);
+        const v3<double> bs = rot_geo_to_gsm(E.A, bg);
         tot.x += bs.x;
         tot.y += bs.y;
         tot.z += bs.z;
     }
     if (EXT == EXT_T96) {
-        const v3<double> bs = t96_field<double>(E->t96d, g.x, g.y, g.z);
+        const v3<double> bs = t96_field<double>(E.t96, g.x, g.y, g.z);
         tot.x += bs.x;
         tot.y += bs.y;
         tot.z += bs.z;
     }
     static_assert(EXT != EXT_TS05 && CRU == CRU_NONE, "model not compiled into this tracer");
-    return rot_gsm_to_geo(E->A, tot);
+    return rot_gsm_to_geo(E.A, tot);
 }
 
 // Mixed path: the SH model takes the GEO position directly (no GSM round trip), the
 // external model's GSM field is rotated to GEO in FP32.
 template <int INT, int EXT, int CRU>
-__device__ __forceinline__ v3<float> field_mixed(const EpochDev *__restrict__ E, v3<double> geo_m,
+__device__ __forceinline__ v3<float> field_mixed(const EpochView<float> &E, v3<double> geo_m,
                                                  v3<double> gsm_m) {
     v3<float> tot = mk3(0.f, 0.f, 0.f);
     if (INT == INT_IGRF) {
         const float x = (float)(geo_m.x / K_RE), y = (float)(geo_m.y / K_RE), z = (float)(geo_m.z / K_RE);
-        tot = igrf_geo<float>(E->Gf, E->Hf, E->RECf, x, y, z);
+        tot = igrf_geo<float>(E.G, E.H, E.REC, x, y, z);
     }
     if (EXT == EXT_T96) {
         const float x = (float)(gsm_m.x / K_RE), y = (float)(gsm_m.y / K_RE), z = (float)(gsm_m.z / K_RE);
-        const v3<float> bg = rot_gsm_to_geo_f(E->Af, t96_field<float>(E->t96f, x, y, z));
+        const v3<float> bg = rot_gsm_to_geo_f(E.Af, t96_field<float>(E.t96, x, y, z));
         tot.x += bg.x;
         tot.y += bg.y;
         tot.z += bg.z;
@@ -207,6 +253,176 @@ __device__ __forceinline__ int find_scan(const ScanDev *__restrict__ scans, int 
     return lo;
 }
 
+// src/trajectory.c:81,91-112: exit point (car_to_sph) and asymptotic direction of an
+// allowed trajectory, written to its result record.
+__device__ __forceinline__ void write_exit(const TraceArgs &a, int64_t gid, v3<double> pos, v3<double> vel,
+                                           double v, double r, double time, double alength) {
+    const double rho2 = pos.x * pos.x + pos.y * pos.y;
+    double th = 0., ph = 0.;
+    if (rho2 != 0.) {
+        ph = atan2(pos.y, pos.x);
+        th = atan2(sqrt(rho2), pos.z);
+        if (ph < 0.) ph += K_2PI;
+    } else if (pos.z < 0.) {
+        th = K_PI;
+    }
+    const double st = sin(th), ct = cos(th), sf = sin(ph), cf = cos(ph);
+    const double be = vel.x * cf + vel.y * sf;
+    const double vr = be * st + vel.z * ct;
+    const double vth = be * ct - vel.z * st;
+    const double vph = -vel.x * sf + vel.y * cf;
+    const double va = vth * ct + vr * st;
+    const double dm = sqrt(vth * vth + va * va);
+    double asf = (ph + atan2(vph, va)) / K_TO_RAD;
+    if (asf > 360.) asf -= 360.;
+    a.out.vc[gid] = v / K_C;
+    a.out.r[gid] = r;
+    a.out.td[gid] = 90. - (th / K_TO_RAD);
+    a.out.fei[gid] = ph / K_TO_RAD;
+    a.out.ast[gid] = atan2(-vth * st + vr * ct, dm) / K_TO_RAD;
+    a.out.asf[gid] = asf;
+    a.out.time[gid] = time;
+    a.out.length[gid] = alength;
+}
+
+// src/trajectory.c:46-152 for one lane, FP64 and in the reference's operation order:
+// RK attempts with the frozen field are repeated until one is accepted (returns -1) or
+// the trajectory ends (returns its status; an allowed trajectory also writes its exit
+// record).  The speed carried into the next attempt is the last attempt's, rejected or
+// not, as in the reference.
+template <class EV>
+__device__ __forceinline__ int attempts_fp64(const TraceArgs &a, const EV &E, v3<double> fld,
+                                             v3<double> &pos, v3<double> &vel, double &v, double b,
+                                             double zn, int &nk0, int nk1, unsigned int &step,
+                                             uint32_t &attempts, double &time, double &alength, int64_t gid) {
+    for (;;) {
+        step++;
+        const double vc = v / K_C;
+        if (vc > 1.) return ST_FORBIDDEN_REENTRY;
+        const double hm = a.mass / sqrt(1. - vc * vc);
+        const double period = (2. * K_PI * hm) / (fabs(zn) * K_Q * b);
+        const double h = period / nk0;
+        const double d = (1.E-09 * zn * K_Q) / hm;
+        const v3<double> pos0 = pos, vel0 = vel;
+        const double vprev = v;
+        rk_step_fp64(h, d, fld, vel, pos);
+        attempts++;
+        v = norm3(vel);
+        if (__builtin_isnan(v)) {
+            // a NaN state meets no termination test: the reference runs it to the step
+            // limit and classifies it as forbidden there
+            step = (unsigned int)(a.step_limit + 1);
+            return ST_FORBIDDEN_LIMIT;
+        }
+        const double gc = (vel.x * vel0.x + vel.y * vel0.y + vel.z * vel0.z) / (v * vprev);
+        bool reject = false;
+        if (gc < 1.) {
+            const double gu = atan2(sqrt((1. - gc) * (1. + gc)), gc);
+            reject = gu >= a.tu_angle;
+        }
+        if (reject && nk0 <= 50000) {
+            nk0 = (int)((unsigned)nk0 * 2u);   // wraps like the reference's int
+            pos = pos0;
+            vel = vel0;
+            step--;
+            continue;
+        }
+        const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
+        const double pax = (gsm.x / K_RE + 25.3) / 36.08;
+        const double paz = (gsm.y * gsm.y + gsm.z * gsm.z) / (K_RE * K_RE);
+        const double r = norm3(pos) / K_RE;
+        const double paus = pax * pax + paz / 459.6736;
+        if (r >= 25. || paus > 1.) {
+            if ((r >= 25. && fabs(r - 25.) < 0.002) || (r <= 25. && paus > 1. && fabs(paus - 1.) < 0.002)) {
+                write_exit(a, gid, pos, vel, v, r, time, alength);
+                return ST_ALLOWED;
+            }
+            nk0 = (int)((unsigned)nk0 * 2u);
+            pos = pos0;
+            vel = vel0;
+            step--;
+            continue;
+        }
+        if (r < 1.) return ST_FORBIDDEN_REENTRY;
+        if ((uint64_t)step > a.step_limit) return ST_FORBIDDEN_LIMIT;
+        // accepted, continuing: src/trajectory.c:150-152
+        time = time + period / nk0;
+        const double dx = pos.x - pos0.x, dy = pos.y - pos0.y, dz = pos.z - pos0.z;
+        alength += sqrt(dx * dx + dy * dy + dz * dz);
+        nk0 = nk1;
+        return -1;
+    }
+}
+
+// The same step control for the mixed path: FP32 RK increments on the FP64 state, all
+// control arithmetic in FP64 but with reciprocal multiplications instead of the
+// reference's divisions, the turn test as gc <= cos(tu_angle) (acos is decreasing),
+// and the sphere tests on |x|^2 with a square root only near the escape radius.
+template <class EV>
+__device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v3<float> fld,
+                                              v3<double> &pos, v3<double> &vel, double &v, double b,
+                                              double zn, int &nk0, int nk1, unsigned int &step,
+                                              uint32_t &attempts, double &time, double &alength, int64_t gid) {
+    constexpr double inv_c = 1.0 / K_C, inv_re = 1.0 / K_RE, inv_re2 = 1.0 / (K_RE * K_RE);
+    constexpr double re2 = K_RE * K_RE, r25sq = (25. * K_RE) * (25. * K_RE);
+    const double inv_mass = 1.0 / a.mass;
+    const double kt = (2. * K_PI) / (fabs(zn) * K_Q);   // period = kt * hm / b
+    const double kd = 1.E-09 * zn * K_Q;                 // d = kd / hm
+    const double inv_b = 1.0 / b;
+    for (;;) {
+        step++;
+        const double vc = v * inv_c;
+        if (vc > 1.) return ST_FORBIDDEN_REENTRY;
+        const double inv_hm = sqrt(1. - vc * vc) * inv_mass;
+        const double hm = 1.0 / inv_hm;
+        const double period = kt * hm * inv_b;
+        const double h = period / nk0;
+        const double d = kd * inv_hm;
+        const v3<double> pos0 = pos, vel0 = vel;
+        const double vprev = v;
+        rk_step_mixed(h, d, fld, vel, pos);
+        attempts++;
+        const double v2 = vel.x * vel.x + vel.y * vel.y + vel.z * vel.z;
+        v = sqrt(v2);
+        if (__builtin_isnan(v)) {
+            step = (unsigned int)(a.step_limit + 1);
+            return ST_FORBIDDEN_LIMIT;
+        }
+        const double dot = vel.x * vel0.x + vel.y * vel0.y + vel.z * vel0.z;
+        if (dot <= a.cos_tu * (v * vprev) && nk0 <= 50000) {   // acos(gc) >= tu_angle
+            nk0 = (int)((unsigned)nk0 * 2u);
+            pos = pos0;
+            vel = vel0;
+            step--;
+            continue;
+        }
+        const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
+        const double pax = (gsm.x * inv_re + 25.3) * (1.0 / 36.08);
+        const double paz = (gsm.y * gsm.y + gsm.z * gsm.z) * inv_re2;
+        const double paus = pax * pax + paz * (1.0 / 459.6736);
+        const double rr2 = pos.x * pos.x + pos.y * pos.y + pos.z * pos.z;
+        if (rr2 >= r25sq || paus > 1.) {
+            const double r = sqrt(rr2) * inv_re;
+            if ((r >= 25. && fabs(r - 25.) < 0.002) || (r <= 25. && paus > 1. && fabs(paus - 1.) < 0.002)) {
+                write_exit(a, gid, pos, vel, v, r, time, alength);
+                return ST_ALLOWED;
+            }
+            nk0 = (int)((unsigned)nk0 * 2u);
+            pos = pos0;
+            vel = vel0;
+            step--;
+            continue;
+        }
+        if (rr2 < re2) return ST_FORBIDDEN_REENTRY;
+        if ((uint64_t)step > a.step_limit) return ST_FORBIDDEN_LIMIT;
+        time = time + h;
+        const double dx = pos.x - pos0.x, dy = pos.y - pos0.y, dz = pos.z - pos0.z;
+        alength += sqrt(dx * dx + dy * dy + dz * dz);
+        nk0 = nk1;
+        return -1;
+    }
+}
+
 template <int PREC>
 struct FieldT;
 template <>
@@ -226,7 +442,8 @@ template <int PREC, int INT, int EXT, int CRU>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COR_WAVES_PER_EU)))
 trace_kernel(TraceArgs a) {
     typedef typename FieldT<PREC>::T FT;
-    const EpochDev *__restrict__ E = a.epoch;
+    __shared__ FT s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
+    const EpochView<FT> E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
     const int lane = __lane_id();
 
     bool active = false;
@@ -282,13 +499,12 @@ trace_kernel(TraceArgs a) {
             // ---- 2. field at the current point (src/trajectory.c:37-38,147-148); the
             // GSM position is recomputed from the GEO state (same arithmetic as the
             // reference's geo_to_gsm after an accepted step)
-            const v3<double> gsm = rot_geo_to_gsm(E->A, pos);
-            if (PREC == PREC_FP64) {
+            const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
+            if constexpr (PREC == PREC_FP64) {
                 const v3<double> f = field_fp64<INT, EXT, CRU>(E, gsm);
                 fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
                 b = 1.E-09 * sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
             } else {
-                fld = mk3<FT>(0, 0, 0);
                 const v3<float> f = field_mixed<INT, EXT, CRU>(E, pos, gsm);
                 fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
                 const double fx = f.x, fy = f.y, fz = f.z;
@@ -296,110 +512,11 @@ trace_kernel(TraceArgs a) {
             }
 
             // ---- 3. attempts until one is accepted or the trajectory ends
-            int status = -1;
-            for (;;) {
-                step++;
-                const double vc = v / K_C;
-                if (vc > 1.) {
-                    status = ST_FORBIDDEN_REENTRY;
-                    break;
-                }
-                const double hm = a.mass / sqrt(1. - vc * vc);
-                const double period = (2. * K_PI * hm) / (fabs(zn) * K_Q * b);
-                const double h = period / nk0;
-                const double d = (1.E-09 * zn * K_Q) / hm;
-                const v3<double> pos0 = pos, vel0 = vel;
-                const double vprev = v;
-                if (PREC == PREC_FP64) {
-                    rk_step_fp64(h, d, mk3<double>(fld.x, fld.y, fld.z), vel, pos);
-                } else {
-                    rk_step_mixed(h, d, mk3<float>(fld.x, fld.y, fld.z), vel, pos);
-                }
-                attempts++;
-                v = norm3(vel);
-                if (__builtin_isnan(v)) {
-                    // a NaN state never meets a termination test: the reference runs it
-                    // to the step limit and classifies it as forbidden there
-                    step = (unsigned int)(a.step_limit + 1);
-                    status = ST_FORBIDDEN_LIMIT;
-                    break;
-                }
-                const double gc = (vel.x * vel0.x + vel.y * vel0.y + vel.z * vel0.z) / (v * vprev);
-                bool reject;
-                if (PREC == PREC_FP64) {
-                    reject = false;
-                    if (gc < 1.) {
-                        const double gu = atan2(sqrt((1. - gc) * (1. + gc)), gc);
-                        reject = gu >= a.tu_angle;
-                    }
-                } else {
-                    reject = gc <= a.cos_tu;   // acos(gc) >= tu_angle
-                }
-                if (reject && nk0 <= 50000) {
-                    nk0 = (int)((unsigned)nk0 * 2u);   // wraps like the reference's int
-                    pos = pos0;
-                    vel = vel0;
-                    step--;
-                    continue;
-                }
-                const v3<double> gsm = rot_geo_to_gsm(E->A, pos);
-                const double pax = (gsm.x / K_RE + 25.3) / 36.08;
-                const double paz = (gsm.y * gsm.y + gsm.z * gsm.z) / (K_RE * K_RE);
-                const double r = norm3(pos) / K_RE;
-                const double paus = pax * pax + paz / 459.6736;
-                if (r >= 25. || paus > 1.) {
-                    if ((r >= 25. && fabs(r - 25.) < 0.002) ||
-                        (r <= 25. && paus > 1. && fabs(paus - 1.) < 0.002)) {
-                        status = ST_ALLOWED;
-                        // src/trajectory.c:81,91-112: exit point and asymptotic direction
-                        const double rho2 = pos.x * pos.x + pos.y * pos.y;
-                        double th = 0., ph = 0.;
-                        if (rho2 != 0.) {
-                            ph = atan2(pos.y, pos.x);
-                            th = atan2(sqrt(rho2), pos.z);
-                            if (ph < 0.) ph += K_2PI;
-                        } else if (pos.z < 0.) {
-                            th = K_PI;
-                        }
-                        const double st = sin(th), ct = cos(th), sf = sin(ph), cf = cos(ph);
-                        const double be = vel.x * cf + vel.y * sf;
-                        const double vr = be * st + vel.z * ct;
-                        const double vth = be * ct - vel.z * st;
-                        const double vph = -vel.x * sf + vel.y * cf;
-                        const double va = vth * ct + vr * st;
-                        const double dm = sqrt(vth * vth + va * va);
-                        double asf = (ph + atan2(vph, va)) / K_TO_RAD;
-                        if (asf > 360.) asf -= 360.;
-                        a.out.vc[gid] = v / K_C;
-                        a.out.r[gid] = r;
-                        a.out.td[gid] = 90. - (th / K_TO_RAD);
-                        a.out.fei[gid] = ph / K_TO_RAD;
-                        a.out.ast[gid] = atan2(-vth * st + vr * ct, dm) / K_TO_RAD;
-                        a.out.asf[gid] = asf;
-                        a.out.time[gid] = time;
-                        a.out.length[gid] = alength;
-                        break;
-                    }
-                    nk0 = (int)((unsigned)nk0 * 2u);   // wraps like the reference's int
-                    pos = pos0;
-                    vel = vel0;
-                    step--;
-                    continue;
-                }
-                if (r < 1.) {
-                    status = ST_FORBIDDEN_REENTRY;
-                    break;
-                }
-                if ((uint64_t)step > a.step_limit) {
-                    status = ST_FORBIDDEN_LIMIT;
-                    break;
-                }
-                // accepted, continuing: src/trajectory.c:150-152
-                time = time + period / nk0;
-                const double dx = pos.x - pos0.x, dy = pos.y - pos0.y, dz = pos.z - pos0.z;
-                alength += sqrt(dx * dx + dy * dy + dz * dz);
-                nk0 = nk1;
-                break;
+            int status;
+            if constexpr (PREC == PREC_FP64) {
+                status = attempts_fp64(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
+            } else {
+                status = attempts_mixed(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
             }
             if (status >= 0) {
                 a.out.status[gid] = status;

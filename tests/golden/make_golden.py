@@ -1,0 +1,98 @@
+"""Write the golden fixtures of the CPU parity tests from the reference, compiled from
+its own sources by oracle/Makefile (oracle/_ref/libcorref.so and oracle/_ref/cor).
+
+The reference ships no tests or fixtures of its own, so these vectors are the pins:
+inputs and the reference's outputs, nothing else.  Run in the container that has the
+reference sources (make -C oracle ref first):
+
+    python tests/golden/make_golden.py
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent))
+import oracle_lib as L  # noqa: E402
+
+EPOCHS = [(2020, 1, 1, 1, 0, 0, 0), (2020, 3, 15, 75, 12, 30, 5), (1965, 6, 1, 152, 3, 0, 0),
+          (1900, 1, 1, 1, 0, 0, 0), (2027, 7, 1, 182, 0, 0, 0), (2029, 12, 31, 365, 23, 59, 59),
+          (1903, 2, 19, 50, 6, 0, 0)]
+T96_PARAMS = [[2, -20, 0, -2], [3, -100, 2, -5], [0.5, 0, 5, 5], [10, -300, -3, 4]]
+
+
+def sh_points(rng, n):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = rng.uniform(1.0, 30.0, size=(n, 1))
+    pts = d * r
+    # poles, the z axis, and radii on both sides of the integer truncation radii
+    extra = [[0, 0, 1.5], [0, 0, -2.5], [1e-7, 0, 3.0], [0, 0, 1.0], [1e-6, 1e-6, -7.0]]
+    for rr in (2.0, 3.0, 4.0, 9.0, 14.0, 29.0):
+        for eps in (-1e-9, 1e-9):
+            extra.append([0.3 * (rr + eps), 0.4 * (rr + eps), np.sqrt(0.75) * (rr + eps)])
+    return np.vstack([pts, np.array(extra)])
+
+
+def t96_points(rng, n):
+    p = np.column_stack([rng.uniform(-40, 15, n), rng.uniform(-20, 20, n), rng.uniform(-20, 20, n)])
+    p = p[np.linalg.norm(p, axis=1) > 1.0]
+    # along the noon-midnight meridian and the equator: Region-1 band edges, Region-2
+    # xi bands, the magnetopause blend, the tail sheet
+    x = np.linspace(-40, 12, 400)
+    lines = [np.column_stack([x, np.zeros_like(x), np.full_like(x, z)]) for z in (0.3, 2.0, 5.0, -3.0)]
+    lines += [np.column_stack([np.full_like(x, xx), np.linspace(-15, 15, 400), np.full_like(x, 1.0)])
+              for xx in (-8.0, 0.5, 6.0)]
+    th = np.linspace(0.01, np.pi - 0.01, 400)
+    for r in (2.0, 3.5, 6.0):
+        lines.append(np.column_stack([r * np.sin(th), np.zeros_like(th), r * np.cos(th)]))
+        lines.append(np.column_stack([-r * np.sin(th), np.zeros_like(th), r * np.cos(th)]))
+        lines.append(np.column_stack([r * np.sin(th) * 0.6, r * np.sin(th) * 0.8, r * np.cos(th)]))
+    return np.vstack([p] + lines)
+
+
+def main():
+    ref = L.RefLib()
+    rng = np.random.default_rng(20240501)
+    out = {}
+    for k, e in enumerate(EPOCHS):
+        rc, p = ref.recalc(*e)
+        assert rc == 0
+        for f in ("G", "H", "REC", "A1", "A2", "A3"):
+            out[f"recalc{k}_{f}"] = np.array(getattr(p, f))
+    out["recalc_epochs"] = np.array(EPOCHS)
+    _, p2020 = ref.recalc(*EPOCHS[0])
+    pts = sh_points(rng, 3000)
+    out["igrf_pts"] = pts
+    out["igrf_b"] = ref.igrf_geo(p2020, pts)
+    tp = t96_points(rng, 4000)
+    out["t96_pts"] = tp
+    for k, pm in enumerate(T96_PARAMS):
+        out[f"t96_params{k}"] = np.array(pm, dtype=float)
+        out[f"t96_b{k}"] = ref.t96(pm, tp)
+    np.savez_compressed(HERE / "field_vectors.npz", **out)
+
+    # cutoff runs of the reference CLI
+    runs = {
+        "config1_igrf_nk100": dict(site=(1.002439629, 46.808601777, 8.0), nk1=100, rig=0.01, dl=0.01,
+                                    rk=20.0, external=None, params=None),
+        "t96_equator_nk400": dict(site=(1.0031392868, 0.0, 0.0), nk1=400, rig=0.05, dl=0.05, rk=20.0,
+                                  external="t96", params=[-20, 2, 0, -2]),
+        "t96_mid_nk400": dict(site=(1.00260, 49.8074, 120.0), nk1=400, rig=0.05, dl=0.05, rk=20.0,
+                              external="t96", params=[-20, 2, 0, -2]),
+        "t96_storm_limit_nk100": dict(site=(1.00280, -35.0, 290.0), nk1=100, rig=1.0, dl=0.25, rk=20.0,
+                                      external="t96", params=[-150, 6, 4, -8], steps=20000),
+    }
+    for name, r in runs.items():
+        r0, lat, lon = r["site"]
+        inp = HERE / f"{name}.in"
+        L.write_cutoff_input(inp, r["rig"], -1.0, r["rk"], r0, lat, lon, lat, lon, nk1=r["nk1"], dl=r["dl"],
+                             ext_params=r["params"])
+        txt = L.run_ref_cutoff(inp, internal="igrf14", external=r["external"], steps=r.get("steps", 1000000))
+        (HERE / f"{name}.out").write_text(txt)
+        print(name, txt.strip().splitlines()[-1])
+
+
+if __name__ == "__main__":
+    main()

