@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--nk1", type=int, default=400)
     ap.add_argument("--step-limit", type=int, default=1000000)
     ap.add_argument("--queue-order", type=int, default=1)
+    ap.add_argument("--tail-lanes", type=int, default=0,
+                    help="cooperative tail threshold (0 = library default, -1 = per-lane mode only)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -122,7 +124,7 @@ def main():
     params = A.recalc(*EPOCH, parmod=PARMOD)
     models = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
     mixed = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_MIXED,
-                   queue_order=args.queue_order)
+                   queue_order=args.queue_order, tail_lanes=args.tail_lanes)
     n = mixed.n
     print(f"[bench] {len(scans)} sites, {n} trajectories; warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
@@ -189,7 +191,7 @@ def main():
                                "(Pdyn 2, Dst -20, By 0, Bz -2), 0.01-20 GV step 0.01",
                    "sites": len(scans), "site_stride": args.stride, "trajectories": int(n),
                    "nk1": args.nk1, "step_limit": args.step_limit, "tu_angle": 0.01,
-                   "queue_order": args.queue_order},
+                   "queue_order": args.queue_order, "tail_lanes": args.tail_lanes},
         "kernel_ms": st_m.kernel_ms,
         "integration_steps_per_s": st_m.steps / (st_m.kernel_ms * 1e-3),
         "fp64": fp64,

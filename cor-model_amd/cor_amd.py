@@ -17,7 +17,7 @@ NCOEF = 105
 DP = C.POINTER(C.c_double)
 
 COR_OK, COR_E_INVALID, COR_E_NO_DEVICE, COR_E_HIP, COR_E_UNSUPPORTED, COR_E_NOMEM = 0, -1, -2, -3, -4, -5
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_WAVE = 0, 1, 2
 INTERNAL_NONE, INTERNAL_IGRF = 0, 1
 EXTERNAL_NONE, EXTERNAL_T96, EXTERNAL_TS05 = 0, 1, 2
 CRUSTAL_NONE, CRUSTAL_SH = 0, 1
@@ -64,7 +64,8 @@ class Scan(C.Structure):
 
 class TraceOpts(C.Structure):
     _fields_ = [("step_limit", C.c_uint64), ("tu_angle", C.c_double), ("particle_mass", C.c_double),
-                ("precision", C.c_int32), ("queue_order", C.c_int32)]
+                ("precision", C.c_int32), ("queue_order", C.c_int32), ("tail_lanes", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class Cutoff(C.Structure):
@@ -196,9 +197,9 @@ class Plan:
     """cor_trace_plan: epoch + scans uploaded to HBM; run() traces them all."""
 
     def __init__(self, models, params, scans, step_limit=1000000, tu_angle=0.01, mass=M_PROTON,
-                 precision=PREC_FP64, queue_order=0):
+                 precision=PREC_FP64, queue_order=0, tail_lanes=0):
         self.scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
-        opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order)
+        opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, tail_lanes, 0)
         h = C.c_void_p()
         self._models = models
         check(lib().cor_trace_plan_create(C.byref(models), C.byref(params), self.scans.ctypes.data,

@@ -1,0 +1,829 @@
+// coop_dev.hpp -- cooperative mode of the mixed tracer: one wavefront per trajectory.
+//
+// Cutoff scans have heavy-tailed step counts: a few trapped trajectories (lowest
+// rigidities) run 1e5..1e6 steps, hundreds of times the median.  In the per-lane mode a
+// wavefront's step takes as long with one live lane as with 64, so once the queue is
+// drained the run time is the latency of those few trajectories.  In cooperative mode
+// the 64 lanes of a wavefront share one trajectory:
+//   * the trajectory state is held by every lane (uniform values, identical arithmetic);
+//   * the field model is split into families of terms of identical form -- Cartesian
+//     harmonics, dipoles, current loops, conical harmonics, conical sheets, cylindrical
+//     harmonics, the tail-87 sheets, the Region-2 sheet terms, IGRF orders -- one term
+//     per lane, so a family is a single SIMD pass whatever its size;
+//   * each lane accumulates its terms; one cross-lane sum per step (DPP, read back from
+//     one lane) gives every lane the same bits;
+//   * the RK attempts and step control are the per-lane mode's code, run redundantly.
+// The region switches of T96 (magnetopause blend, Region-1 bands, Region-2 xi bands) are
+// uniform per step, so no family pass diverges on them.
+//
+// Arithmetic is FP32 like the per-lane mixed path.  The interconnection harmonics (a fit
+// whose terms cancel by four orders of magnitude) are summed in a form without the
+// cancellation (see the lane-table builder).
+#pragma once
+#include "dev_types.hpp"
+#include "igrf_dev.hpp"
+#include "t96_dev.hpp"
+
+namespace cor {
+
+// ---------------------------------------------------------------- lane-table builder
+
+// Per-epoch lane tables from the T96 coefficient tables (lib/t96/t96.c function-level
+// citations at each family) and the epoch's amplitudes.  One thread; runs once per plan.
+__device__ inline void co_circ(CoopTab &T, int &l, const double R[9], double cx, double cy, double cz,
+                               double rl, double coef, int group) {
+    for (int k = 0; k < 9; k++) T.circ[l][k] = (float)R[k];
+    T.circ[l][9] = (float)cx;
+    T.circ[l][10] = (float)cy;
+    T.circ[l][11] = (float)cz;
+    T.circ[l][12] = (float)rl;
+    T.circ[l][13] = (float)coef;
+    T.circ[l][14] = (float)group;
+    T.circ[l][15] = 0.f;
+    l++;
+}
+
+// crosslp (lib/t96/t96.c:883-898): two loops tilted by +-alpha about the x axis
+__device__ inline void co_crosslp(CoopTab &T, int &l, double xc, double rl, double cal, double sal,
+                                  double coef, int group) {
+    const double R1[9] = {1, 0, 0, 0, cal, -sal, 0, sal, cal};
+    const double R2[9] = {1, 0, 0, 0, cal, sal, 0, -sal, cal};
+    co_circ(T, l, R1, xc, 0, 0, rl, coef, group);
+    co_circ(T, l, R2, xc, 0, 0, rl, coef, group);
+}
+
+// loops4 (lib/t96/t96.c:1414-1470): four mirrored loops, each rotated about z by the
+// azimuth and then tilted about y; local = Tilt * Rz * (p - centre)
+__device__ inline void co_loops4(CoopTab &T, int &l, double xc, double yc, double zc, double rl, double st,
+                                 double ct, double sp, double cp, double coef, int group) {
+    const double M[4][9] = {{cp, sp, 0, -sp, cp, 0, 0, 0, 1},
+                            {cp, -sp, 0, sp, cp, 0, 0, 0, 1},
+                            {-cp, sp, 0, -sp, -cp, 0, 0, 0, 1},
+                            {-cp, -sp, 0, sp, -cp, 0, 0, 0, 1}};
+    const double C[4][3] = {{xc, yc, zc}, {xc, -yc, zc}, {xc, -yc, -zc}, {xc, yc, -zc}};
+    const double Tt[9] = {ct, 0, -st, 0, 1, 0, st, 0, ct};
+    for (int j = 0; j < 4; j++) {
+        double R[9];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                R[3 * r + c] = Tt[3 * r] * M[j][c] + Tt[3 * r + 1] * M[j][3 + c] + Tt[3 * r + 2] * M[j][6 + c];
+        co_circ(T, l, R, C[j][0], C[j][1], C[j][2], rl, coef, group);
+    }
+}
+
+__device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
+    using namespace t96;
+    float *z = &T.harm[0][0];
+    for (size_t k = 0; k < sizeof(CoopTab) / sizeof(float); k++) z[k] = 0.f;
+    const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+
+    // Cartesian harmonics: tailrc_96 shield sets (lib/t96/t96.c:501-617), birk1shld
+    // (:1030-1194), birk2shl (:1196-1263)
+    int l = 0;
+    const double set_amp[3] = {P.rcampl, P.tampl2, P.tampl3};
+    for (int set = 0; set < 3; set++) {
+        const double *tab = k_shl_d + 15 * set;
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++, l++) {
+                const double p = tab[9 + i], r = tab[12 + k];
+                T.harm[l][0] = (float)(1.0 / p);
+                T.harm[l][1] = (float)(1.0 / r);
+                T.harm[l][2] = (float)sqrt(1.0 / (p * p) + 1.0 / (r * r));
+                T.harm[l][3] = (float)(tab[3 * i + k] * set_amp[set]);
+            }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++, l++) {
+            const double rp = 1.0 / k_b1s_py_d[i], rk = 1.0 / k_b1s_pz_d[k];
+            T.harm[l][0] = (float)rp;
+            T.harm[l][1] = (float)rk;
+            T.harm[l][2] = (float)sqrt(rp * rp + rk * rk);
+            T.harm[l][3] = (float)(k_b1s_amp_d[4 * i + k] * P.b1ampl);
+        }
+    {
+        const double ai[2] = {13.85650567, 14.90554500}, ak[2] = {10.21914434, 10.09021632};
+        const double amp[4] = {-111.6371348 + 124.5402702, 110.3735178 + -122.0095905,
+                               111.9448247 + -129.1957743, -110.7586562 + 126.5649012};
+        for (int i = 0; i < 2; i++)
+            for (int k = 0; k < 2; k++, l++) {
+                T.harm[l][0] = (float)(1.0 / ai[i]);
+                T.harm[l][1] = (float)(1.0 / ak[k]);
+                T.harm[l][2] = (float)sqrt(1.0 / (ai[i] * ai[i]) + 1.0 / (ak[k] * ak[k]));
+                T.harm[l][3] = (float)(amp[2 * i + k] * P.b2ampl);
+            }
+    }
+    // interconnection field (lib/t96/t96.c:236-271).  Per y harmonic i, the k = 0 term
+    // as it is; the k = 1, 2 pair (coefficients ~1e7 on sin(z r_k), r_k ~ 1e-3, whose sum
+    // cancels by four orders of magnitude) as the series
+    //   sum_k c_k s_k e^{x s_k} sin(z r_k)
+    //     = e^{x s_1} (sum_j (-1)^j z^{2j+1} A_{2j+1} / (2j+1)!  +  c_2 s_2 delta sin(z r_2)),
+    //   A_n = c_1 s_1 r_1^n + c_2 s_2 r_2^n,  delta = e^{x (s_2 - s_1)} - 1,
+    // and the same with B_n = c_1 r_1^n + c_2 r_2^n for the y component (sine) and the z
+    // component (cosine, odd n).  The A_n, B_n are formed here in FP64, so the device sums
+    // terms that no longer cancel; |z r_k| < 0.1 makes the n <= 7 series exact in FP32.
+    {
+        const double a[9] = {-8.411078731, 5932254.951, -9073284.93, -11.68794634, 6027598.824,
+                             -9218378.368, -6.508798398, -11824.42793, 18015.66212};
+        const double rp[3] = {0.12503844191332988, 0.071597395204476152, 0.011080976889832743};
+        const double rr[3] = {0.059675527522452816, 0.00098459526668325296, 0.00064371058922777988};
+        for (int i = 0; i < 3; i++) {
+            float *t = T.icon[i];
+            const double c0 = a[3 * i] * P.rimfampl, c1 = a[3 * i + 1] * P.rimfampl, c2 = a[3 * i + 2] * P.rimfampl;
+            const double p2 = rp[i] * rp[i];
+            const double s0 = sqrt(p2 + rr[0] * rr[0]), s1 = sqrt(p2 + rr[1] * rr[1]), s2 = sqrt(p2 + rr[2] * rr[2]);
+            double A[4], B[4];
+            for (int j = 0; j < 4; j++) {
+                const double q1 = pow(rr[1], 2 * j + 1), q2 = pow(rr[2], 2 * j + 1);
+                A[j] = c1 * s1 * q1 + c2 * s2 * q2;
+                B[j] = c1 * q1 + c2 * q2;
+            }
+            const double v[21] = {rp[i], rr[0], s0, c0, s1, (rr[2] * rr[2] - rr[1] * rr[1]) / (s1 + s2), rr[2],
+                                  c2 * s2, c2, c2 * rr[2], A[0], A[1] / 6., A[2] / 120., A[3] / 5040.,
+                                  B[0], B[1] / 6., B[2] / 120., B[3] / 5040., B[1] / 2., B[2] / 24., B[3] / 720.};
+            for (int k = 0; k < 21; k++) t[k] = (float)v[k];
+        }
+    }
+    // conical sheets: ringcurr_96 (lib/t96/t96.c:322-381), taildisk_t96 (:383-443)
+    {
+        const double fr[2] = {569.895366, -1603.386993}, br[2] = {2.722188, 3.766875};
+        const double fd[4] = {-745796.7338, 1176470.141, -444610.529, -57508.01028};
+        const double bd[4] = {7.9250000, 8.0850000, 8.4712500, 27.89500};
+        for (int i = 0; i < 2; i++) {
+            T.cone[i][0] = (float)(fr[i] * P.rcampl);
+            T.cone[i][1] = (float)br[i];
+        }
+        for (int i = 0; i < 4; i++) {
+            T.cone[2 + i][0] = (float)(fd[i] * P.tampl2);
+            T.cone[2 + i][1] = (float)bd[i];
+            T.cone[2 + i][2] = 4.5f;
+            T.cone[2 + i][3] = 10.f;
+        }
+    }
+    // cylindrical harmonics (lib/t96/t96.c:113-163)
+    {
+        const double a[12] = {0.24777, -27.003, -0.46815, 7.0637, -1.5918, -0.90317E-01,
+                              57.522, 13.757, 2.0100, 10.458, 4.5798, 2.1695};
+        for (int i = 0; i < 6; i++) {
+            T.cyl[i][0] = (float)(a[i] * P.xappa3);
+            T.cyl[i][1] = (float)a[i + 6];
+            T.cyl[i][2] = i < 3 ? 0.f : 1.f;
+        }
+    }
+    // tail87 (lib/t96/t96.c:445-499): the sheet at z and its images at z -+ 40
+    T.t87[0][0] = 0.f;
+    T.t87[0][1] = (float)P.tampl3;
+    T.t87[1][0] = -40.f;
+    T.t87[1][1] = (float)(-0.5 * P.tampl3);
+    T.t87[2][0] = 40.f;
+    T.t87[2][1] = (float)(-0.5 * P.tampl3);
+    // dipoles: diploop1 (lib/t96/t96.c:809-852) then condip1 (:920-1028), each a
+    // point dipole of moment M at position D: field 30574 (3 (M.r) r - r^2 M) / r^5
+    {
+        int s = 0;
+        auto dip = [&](double dx, double dy, double dz, double mx, double my, double mz) {
+            T.dip[s][0] = (float)dx;
+            T.dip[s][1] = (float)dy;
+            T.dip[s][2] = (float)dz;
+            T.dip[s][4] = (float)mx;
+            T.dip[s][5] = (float)my;
+            T.dip[s][6] = (float)mz;
+            s++;
+        };
+        for (int i = 0; i < 12; i++) {
+            const double xd = k_dl_xx_d[i] * 1.12541, yd = k_dl_yy_d[i] * 0.945719;
+            const double c = k_dl_c1_d[i] * P.b1ampl;
+            dip(xd, yd, 0., 0., 0., c);
+            if (k_dl_yy_d[i] != 0.) dip(xd, -yd, 0., 0., 0., c);
+        }
+        for (int i = 0; i < 9; i++) {
+            const double xd = k_cd_xd_d[i], yd = k_cd_yd_d[i], zd = k_cd_zz_d[i];
+            const int ix = (i + 1) * 3 + 2;
+            const double cx = k_cd_c2_d[ix] * P.b1ampl, cy = k_cd_c2_d[ix + 1] * P.b1ampl,
+                         cz = k_cd_c2_d[ix + 2] * P.b1ampl;
+            dip(xd, yd, zd, cx, cy, cz);
+            dip(xd, -yd, zd, cx, -cy, cz);
+            dip(xd, yd, -zd, -cx, -cy, cz);
+            dip(xd, -yd, -zd, -cx, cy, cz);
+        }
+        for (int i = 1; i <= 5; i++) {
+            const double zd = k_cd_zz_d[i + 8];
+            const int ix = 30 + i * 2;
+            const double cx = k_cd_c2_d[ix] * P.b1ampl, cz = k_cd_c2_d[ix + 1] * P.b1ampl;
+            dip(0., 0., zd, cx, 0., cz);
+            dip(0., 0., -zd, -cx, 0., cz);
+        }
+    }
+    // current loops: diploop1's crossed pair and circle (group 0), r2outer (group 1,
+    // lib/t96/t96.c:1388-1412), r2inner's loops4 (group 2, :1323-1366)
+    {
+        int c = 0;
+        co_crosslp(T, c, 2.28397, 1.86106, 0.5327834519503802, 0.8462516134861054, k_dl_c1_d[12] * P.b1ampl, 0);
+        co_circ(T, c, I3, -5.60831, 0., 0., 7.83281, k_dl_c1_d[13] * P.b1ampl, 0);
+        co_crosslp(T, c, 0.55, 0.694, 0.999995195003848, 0.003099995034835719, -34.105 * P.b2ampl, 1);
+        co_crosslp(T, c, 1.55, 2.8, 0.9905617592124779, 0.13706714116038543, -2.00019 * P.b2ampl, 1);
+        co_crosslp(T, c, -0.7, 0.2, 0.5714702170359919, 0.8206228068003206, 628.639 * P.b2ampl, 1);
+        co_circ(T, c, I3, -2.994, 0., 0., 2.925, 73.4847 * P.b2ampl, 1);
+        co_loops4(T, c, -1.775, 4.3, -0.275, 2.7, 0.4179612609200968, 0.9084648503767676, 0.999783764189357,
+                  0.020794827803092428, 12.5162 * P.b2ampl, 1);
+        co_loops4(T, c, -8.1902, 6.5239, 5.504, 7.7815, 0.7560782215370907, 0.6544812624646408,
+                  0.04297941043769308, -0.9990759582126018, 99.9132 * P.b2ampl, 2);
+    }
+    // conical harmonics: condip1's (group 0, at x + 0.16) and r2inner's bconic (group 1)
+    {
+        const double cc[5] = {154.185, -2.12446, 0.601735E-01, -.153954E-02, .355077E-04};
+        for (int m = 1; m <= 5; m++) {
+            T.conic[m - 1][0] = (float)m;
+            T.conic[m - 1][1] = (float)(k_cd_c2_d[m - 1] * P.b1ampl);
+            T.conic[m - 1][2] = 0.16f;
+            T.conic[m - 1][3] = 0.f;
+            T.conic[4 + m][0] = (float)m;
+            T.conic[4 + m][1] = (float)(cc[m - 1] * P.b2ampl);
+            T.conic[4 + m][2] = 0.f;
+            T.conic[4 + m][3] = 1.f;
+        }
+    }
+    // Region-2 sheet (lib/t96/t96.c:1472-1594): per component and latitude term k the
+    // factor alpha (ct)^[comp != z] exp(beta ct^2 + gamma), the t-factor constants and
+    // the 16 coefficients of the four phi harmonics
+    {
+        const double alpha[3][5] = {{10.189284199579348, 7.106076658978449, 0.839673526422457, 1., 1.},
+                                    {8.622355131189718, 6.038125124921339, 1., 1., 1.},
+                                    {1., 1., 1., 1., 1.}};
+        const double beta[3][5] = {{-19.0969, -9.28828, -0.129687, 5.58594, 22.5055},
+                                   {-13.6750, -6.70625, 2.31875, 11.4062, 20.4562},
+                                   {-16.7125, -16.4625, -0.1625, 5.1, 23.7125}};
+        const double gamma[3][5] = {{0., 0., 0., -5.58594, -22.5055},
+                                    {0., 0., -2.31875, -11.4062, -20.4562},
+                                    {0., 0., 0., -5.1, -23.7125}};
+        const double tf[3][4] = {{0.483750e-01, 0.396953e-01, 0.579023e-01, 1.124043867626711e-05},
+                                 {0.478750e-01, 0.363750e-01, 0.567500e-01, 1.0372022816406252e-05},
+                                 {0.355625e-01, 0.318750e-01, 0.538750e-01, 8.42459695336914e-06}};
+        for (int comp = 0; comp < 3; comp++)
+            for (int k = 0; k < 5; k++) {
+                float *s = T.sheet[comp * 5 + k];
+                s[0] = (float)alpha[comp][k];
+                s[1] = (float)beta[comp][k];
+                s[2] = (float)gamma[comp][k];
+                s[3] = (float)comp;
+                for (int j = 0; j < 4; j++) s[4 + j] = (float)tf[comp][j];
+                for (int j = 0; j < 16; j++) s[8 + j] = (float)(k_r2s_d[comp * 80 + 16 * k + j] * P.b2ampl);
+            }
+    }
+}
+
+// ---------------------------------------------------------------- wavefront helpers
+
+// Sum of each of N values over the 64 lanes with DPP row operations (pairs, quads,
+// half rows and rows by permutation, then rows 1/3 and 2/3 by row broadcasts); the
+// total ends in lane 63 and is read from there, so every lane gets the same bits.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+template <int N>
+__device__ __forceinline__ void wave_sum(float *p) {
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = dpp_add<0xB1, 0xF>(p[k]);    // quad_perm [1,0,3,2]
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = dpp_add<0x4E, 0xF>(p[k]);    // quad_perm [2,3,0,1]
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = dpp_add<0x141, 0xF>(p[k]);   // row_half_mirror
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = dpp_add<0x140, 0xF>(p[k]);   // row_mirror
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = dpp_add<0x142, 0xA>(p[k]);   // row_bcast:15 -> rows 1, 3
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = dpp_add<0x143, 0xC>(p[k]);   // row_bcast:31 -> rows 2, 3
+#pragma unroll
+    for (int k = 0; k < N; k++) p[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[k]), 63));
+}
+
+// This lane's rows of the lane tables, read together at the top of a step so their LDS
+// latencies overlap (rows past a family's size are valid zero or unused rows).
+struct CoopLane {
+    float4 harm, cone, cyl, t87, conic;
+    float icon[24], circ[16], sheet[24];
+};
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void put4(float *d, float4 v) {
+    d[0] = v.x;
+    d[1] = v.y;
+    d[2] = v.z;
+    d[3] = v.w;
+}
+
+__device__ __forceinline__ void load_lane(const CoopTab &T, int lane, CoopLane &L) {
+    L.harm = ld4(T.harm[lane]);
+    L.cone = ld4(T.cone[lane & 7]);
+    L.cyl = ld4(T.cyl[lane & 7]);
+    L.t87 = ld4(T.t87[lane & 3]);
+    L.conic = ld4(T.conic[lane & 15]);
+    const int ci = lane < 24 ? lane : 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) put4(L.icon + 4 * j, ld4(&T.icon[lane & 3][4 * j]));
+#pragma unroll
+    for (int j = 0; j < 4; j++) put4(L.circ + 4 * j, ld4(&T.circ[ci][4 * j]));
+#pragma unroll
+    for (int j = 0; j < 6; j++) put4(L.sheet + 4 * j, ld4(&T.sheet[lane & 15][4 * j]));
+}
+
+// One term of the harmonic form shared by the shield sets, birk1shld, birk2shl and the
+// interconnection field: c e^{x sq} (-sq cos(y rp) sin(z rk), rp sin(y rp) sin(z rk),
+// -rk cos(y rp) cos(z rk)).
+template <class T>
+__device__ __forceinline__ v3<T> co_harm(T x, T y, T z, T rp, T rk, T sq, T c) {
+    T sy, cy, sz, cz;
+    Dm<T>::sincos(y * rp, &sy, &cy);
+    Dm<T>::sincos(z * rk, &sz, &cz);
+    const T ce = c * Dm<T>::exp(x * sq);
+    return mk3<T>(-sq * ce * cy * sz, rp * ce * sy * sz, -rk * ce * cy * cz);
+}
+
+// ---------------------------------------------------------------- T96, cooperative
+
+// This lane's share of the sum FX of lib/t96/t96.c:91-93 plus the uniform terms that
+// finish the model (lib/t96/t96.c:4-111).
+struct T96Share {
+    v3<float> fx;       // this lane's terms of FX
+    v3<float> q;        // dipole field (for the magnetopause blend)
+    float sigma, oimfy, oimfz;
+};
+
+__device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, const CoopTab &T, float x, float y,
+                                                   float z, int lane) {
+    using namespace t96;
+    typedef float F;
+    T96Share S;
+    // ---- uniform preamble (lib/t96/t96.c:22-60)
+    const F ys = y * P.ct - z * P.st, zs = z * P.ct + y * P.st;
+    const F ysd = ys / F(10.0);
+    const F factimf = Dm<F>::exp(x / F(20.0) - ysd * ysd);
+    S.oimfy = P.oimfy0 * factimf;
+    S.oimfz = P.oimfz0 * factimf;
+    const F rho2 = y * y + z * z;
+    F xmxm = P.am + x - P.c548;
+    if (xmxm < F(0.0)) xmxm = F(0.0);
+    const F axx0 = xmxm * xmxm;
+    const F aro = P.asq + rho2;
+    const F aro_axx0 = aro + axx0;
+    S.sigma = Dm<F>::sqrt((aro + axx0 + Dm<F>::sqrt(aro_axx0 * aro_axx0 - P.asq4 * axx0)) / P.asq2);
+    S.q = dipole(x, y, z);
+    S.fx = mk3<F>(0, 0, 0);
+    if (S.sigma >= F(1.08 + 0.005)) return S;
+    CoopLane L;
+    load_lane(T, lane, L);
+    const F xx = x * P.xappa, yy = y * P.xappa, zz = z * P.xappa;
+    F px = 0, py = 0, pz = 0;
+
+    // ---- uniform region geometry.  Region 1 (lib/t96/t96.c:619-807): which basis
+    // applies, at which point, with which blend weight.
+    F lx = xx, ly = yy, lz = zz, cxp = xx, cyp = yy, czp = zz, wl = 0, wc = 0;
+    bool need_loop, need_cone;
+    {
+        const F tnoonn = F(12.0 * 0.01745329), tnoons = F(3.141592654 - 12.0 * 0.01745329);
+        const F r2 = xx * xx + yy * yy + zz * zz;
+        const F r = Dm<F>::sqrt(r2);
+        const F r3 = r * r2;
+        F pas = F(0.0);
+        if (xx != F(0.0) || yy != F(0.0)) pas = Dm<F>::atan2(yy, xx);
+        const F tas = Dm<F>::atan2(Dm<F>::sqrt(xx * xx + yy * yy), zz);
+        const F stas = Dm<F>::sin(tas);
+        F tet0 = Dm<F>::asin(stas / Dm<F>::pow((stas * stas * stas * stas * stas * stas) * (F(1.0) - r3) + r3,
+                                                 F(0.1666666667)));
+        if (tas > F(1.5707963)) tet0 = F(3.141592654) - tet0;
+        const F sp2 = Dm<F>::sin(pas * F(0.5));
+        const F dtet = F(8.0 * 0.01745329) * (sp2 * sp2);
+        const F tetr1n = tnoonn + dtet, tetr1s = tnoons - dtet;
+        const F t01n = tetr1n - F(0.034906), t01s = tetr1s - F(0.034906);
+        const F t02s = tetr1s + F(0.034906), t02n = tetr1n + F(0.034906);
+        const bool loop_only = (tet0 < t01n || tet0 > t02s);
+        const bool cone_only = (tet0 > t02n && tet0 < t01s);
+        const bool north = (tet0 >= t01n && tet0 <= t02n);
+        const bool south = (tet0 >= t01s && tet0 <= t02s);
+        need_loop = loop_only || north || south;
+        need_cone = cone_only || north || south;
+        if (north || south) {
+            const F sqr = Dm<F>::sqrt(r);
+            const F ta = north ? t01n : t01s, tb = north ? t02n : t02s;
+            const F sa = Dm<F>::sin(ta), sb = Dm<F>::sin(tb);
+            const F st1 = sqr / Dm<F>::pow(r3 + F(1.0) / (sa * sa * sa * sa * sa * sa) - F(1.0), F(0.1666666667));
+            const F st2 = sqr / Dm<F>::pow(r3 + F(1.0) / (sb * sb * sb * sb * sb * sb) - F(1.0), F(0.1666666667));
+            F ct1 = Dm<F>::sqrt(F(1.0) - st1 * st1), ct2 = Dm<F>::sqrt(F(1.0) - st2 * st2);
+            if (south) {
+                ct1 = -ct1;
+                ct2 = -ct2;
+            }
+            const F sps = Dm<F>::sin(pas), cps = Dm<F>::cos(pas);
+            const F px1 = r * st1 * cps, py1 = r * st1 * sps, pz1 = r * ct1;
+            const F px2 = r * st2 * cps, py2 = r * st2 * sps, pz2 = r * ct2;
+            const F ddx = px2 - px1, ddy = py2 - py1, ddz = pz2 - pz1;
+            const F ss = Dm<F>::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            const F qx = xx - px1, qy = yy - py1, qz = zz - pz1;
+            const F frac = Dm<F>::sqrt(qx * qx + qy * qy + qz * qz) / ss;
+            // loop basis at the first point (north) / second point (south), conical
+            // basis at the other one; the first point's basis weighs 1 - frac
+            lx = south ? px2 : px1;
+            ly = south ? py2 : py1;
+            lz = south ? pz2 : pz1;
+            cxp = north ? px2 : px1;
+            cyp = north ? py2 : py1;
+            czp = north ? pz2 : pz1;
+            wl = north ? F(1.0) - frac : frac;
+            wc = north ? frac : F(1.0) - frac;
+        } else {
+            wl = loop_only ? F(1.0) : F(0.0);
+            wc = cone_only ? F(1.0) : F(0.0);
+        }
+    }
+    // Region 2 (lib/t96/t96.c:1265-1321): weights of the outer, sheet and inner
+    // representations, already times the -0.02 scale of r2birk
+    F wo = 0, ws = 0, wi = 0;
+    const F xks = xksi(xx, yy, zz);
+    {
+        const F del = F(0.03), del1 = F(0.015);
+        if (xks < -(del + del1)) {
+            wo = F(-0.02);
+        } else if (xks >= -(del + del1) && xks < -del + del1) {
+            ws = F(-0.02) * tksi(xks, -del, del1);
+            wo = F(-0.02) - ws;
+        } else if (xks >= -del + del1 && xks < del - del1) {
+            ws = F(-0.02);
+        } else if (xks >= del - del1 && xks < del + del1) {
+            wi = F(-0.02) * tksi(xks, del, del1);
+            ws = F(-0.02) - wi;
+        } else if (xks >= del + del1) {
+            wi = F(-0.02);
+        }
+    }
+
+    // ---- family 1: Cartesian harmonics (47 lanes)
+    {
+        const float4 h = L.harm;
+        const v3<F> b = co_harm<F>(xx, yy, zz, h.x, h.y, h.z, h.w);
+        px += b.x;
+        py += b.y;
+        pz += b.z;
+    }
+    // ---- family 2: interconnection harmonics (3 lanes, one y harmonic each) at the
+    // IMF-rotated point, rotated back (lib/t96/t96.c:66-72)
+    if (lane < CO_NICON) {
+        const float *c = L.icon;
+        const F X = xx, Y = ys * P.xappa, Z = zs * P.xappa;
+        F sy, cy, sz, cz, s2, c2;
+        Dm<F>::sincos(Y * c[0], &sy, &cy);
+        Dm<F>::sincos(Z * c[1], &sz, &cz);
+        Dm<F>::sincos(Z * c[6], &s2, &c2);
+        const F e0 = c[3] * Dm<F>::exp(X * c[2]);
+        const F e1 = Dm<F>::exp(X * c[4]);
+        const F u = X * c[5];
+        const F del = u * (F(1.0) + u * (F(0.5) + u * F(1.0 / 6.0)));
+        const F z2 = Z * Z;
+        const F sx = Z * (c[10] - z2 * (c[11] - z2 * (c[12] - z2 * c[13]))) + c[7] * del * s2;
+        const F sy2 = Z * (c[14] - z2 * (c[15] - z2 * (c[16] - z2 * c[17]))) + c[8] * del * s2;
+        const F sz2 = c[14] - z2 * (c[18] - z2 * (c[19] - z2 * c[20])) + c[9] * del * c2;
+        const F bx = -cy * (c[2] * e0 * sz + e1 * sx);
+        const F by = c[0] * sy * (e0 * sz + e1 * sy2);
+        const F bz = -cy * (c[1] * e0 * cz + e1 * sz2);
+        px += bx;
+        py += by * P.ct + bz * P.st;
+        pz += bz * P.ct - by * P.st;
+    }
+    // ---- family 3: conical sheets of the ring current and the tail disk (6 lanes)
+    if (lane < CO_NCONE) {
+        const float4 c = L.cone;
+        const F f = c.x, beta = c.y, xs = xx - c.z, d1 = c.w;
+        const F y20 = yy / F(20.0);
+        const F dd = F(2.0) + d1 * (y20 * y20);
+        const F dzs = Dm<F>::sqrt(zz * zz + dd * dd);
+        const F ddz_dy = (dd * d1 * yy * F(0.005)) / dzs;
+        const F ddz_dz = zz / dzs;
+        const F rhos = Dm<F>::sqrt(xs * xs + yy * yy);
+        F drx = F(0.0), dry = (yy >= 0) ? F(1.0) : F(-1.0);
+        if (rhos >= F(1.0e-5)) {
+            drx = xs / rhos;
+            dry = yy / rhos;
+        }
+        F as, da1, da2, s1z, s2z, s1r, s2r;
+        cone_terms(dzs + beta, rhos, beta, &as, &da1, &da2, &s1z, &s2z, &s1r, &s2r);
+        const F dasx = da1 * (s1r * drx) + da2 * (s2r * drx);
+        const F dasy = da1 * (s1z * ddz_dy + s1r * dry) + da2 * (s2z * ddz_dy + s2r * dry);
+        const F dasz = da1 * (s1z * ddz_dz) + da2 * (s2z * ddz_dz);
+        px += -f * xs * dasz;
+        py += -f * yy * dasz;
+        pz += f * ((F(2.0) * as + yy * dasy) + xs * dasx);
+    }
+    // ---- family 4: cylindrical harmonics (6 lanes)
+    if (lane < CO_NCYL) {
+        const float4 c = L.cyl;
+        const F a = c.x, scale = c.y;
+        F rho = Dm<F>::sqrt(yy * yy + zz * zz);
+        F sinfi, cosfi;
+        if (rho < F(1.0e-8)) {
+            sinfi = F(1.0);
+            cosfi = F(0.0);
+            rho = F(1.0e-8);
+        } else {
+            sinfi = zz / rho;
+            cosfi = yy / rho;
+        }
+        const F dz = rho / scale, xk = xx / scale;
+        F j0, j1;
+        bessel01(dz, &j0, &j1);
+        const F e = Dm<F>::exp(xk);
+        if (c.z == F(0.0)) {
+            const F sinfi2 = sinfi * sinfi;
+            const F si2co2 = sinfi2 - cosfi * cosfi;
+            px += -a * j1 * e * sinfi;
+            py += a * (F(2.0) * j1 / dz - j0) * e * sinfi * cosfi;
+            pz += a * (j1 / dz * si2co2 - j0 * sinfi2) * e;
+        } else {
+            const F brho = (xk * j0 - (dz * dz + xk - F(1.0)) * j1 / dz) * e * sinfi;
+            const F bphi = (j0 + j1 / dz * (xk - F(1.0))) * e * cosfi;
+            px += a * (dz * j0 + xk * j1) * e * sinfi;
+            py += a * (brho * cosfi - bphi * sinfi);
+            pz += a * (brho * sinfi + bphi * cosfi);
+        }
+    }
+    // ---- family 5: tail87 sheet and images (3 lanes)
+    if (lane < CO_NT87) {
+        const float4 c = L.t87;
+        const F zl = zz + c.x, w = c.y;
+        const F xnx = F(-10.0) - xx, xnx2 = xnx * xnx;
+        const F xc1 = xx + F(1.261), xc2 = xx + F(0.663);
+        const F xc22 = xc2 * xc2, xr2 = xc2 * F(-0.1071), xc12 = xc1 * xc1;
+        const F b2 = zl * zl + F(9.0);
+        const F bb = Dm<F>::sqrt(b2);
+        const F xa1 = xc12 + b2, xa2 = F(1.) / (xc22 + b2), xna = xnx2 + b2;
+        const F ff = b2 - xc22;
+        const F ln1 = Dm<F>::log(F(76.37) / xna);
+        const F ln2 = ln1 + F(0.13238005);
+        const F s0 = (Dm<F>::atan(xnx / bb) + F(1.5707963)) / bb;
+        const F s1 = (ln1 * F(.5) + xc1 * s0) / xa1;
+        const F s2 = (xc2 * xa2 * ln2 + F(0.1071) - ff * xa2 * s0) * xa2;
+        const F g1 = (b2 * s0 - F(0.5) * xc1 * ln1) / xa1;
+        const F g2 = ((F(0.5) * ff * ln2 + F(2.) * s0 * b2 * xc2) * xa2 + xr2) * xa2;
+        px += w * zl * (F(0.391734) * s0 + F(5.89715) * s1 + F(24.6833) * s2);
+        pz += w * (F(-0.5) * F(0.391734) * ln1 + F(5.89715) * g1 + F(24.6833) * g2);
+    }
+    // ---- family 6: Region-1 dipoles (loop basis 21, conical basis 46; 1 or 2 passes)
+    {
+        const int d0 = need_loop ? 0 : CO_NDIP_LOOP;
+        const int d1 = need_cone ? CO_NDIP : CO_NDIP_LOOP;
+        for (int base = d0; base < d1; base += 64) {
+            const int s = base + lane;
+            if (s < d1) {
+                const float4 dp = ld4(&T.dip[s][0]);
+                const float4 mm = ld4(&T.dip[s][4]);
+                const bool isloop = s < CO_NDIP_LOOP;
+                const F w = isloop ? wl : wc;
+                const F X = (isloop ? lx : cxp) - dp.x, Y = (isloop ? ly : cyp) - dp.y, Z = (isloop ? lz : czp) - dp.z;
+                const F r2 = X * X + Y * Y + Z * Z;
+                const F qd = w * F(30574.0) / (r2 * r2 * Dm<F>::sqrt(r2));
+                const F md = F(3.0) * (mm.x * X + mm.y * Y + mm.z * Z);
+                px += qd * (md * X - r2 * mm.x);
+                py += qd * (md * Y - r2 * mm.y);
+                pz += qd * (md * Z - r2 * mm.z);
+            }
+        }
+    }
+    // ---- family 7: current loops (18 lanes; groups weighted by wl, wo, wi)
+    if (lane < CO_NCIRC) {
+        const float *c = L.circ;
+        const int g = (int)c[14];
+        const F w = g == 0 ? wl : (g == 1 ? wo : wi);
+        if (w != F(0.0)) {
+            const F ux = (g == 0 ? lx : xx) - c[9], uy = (g == 0 ? ly : yy) - c[10], uz = (g == 0 ? lz : zz) - c[11];
+            const F qx = c[0] * ux + c[1] * uy + c[2] * uz;
+            const F qy = c[3] * ux + c[4] * uy + c[5] * uz;
+            const F qz = c[6] * ux + c[7] * uy + c[8] * uz;
+            const v3<F> bl = circle(qx, qy, qz, c[12]);
+            const F k = w * c[13];
+            px += k * (c[0] * bl.x + c[3] * bl.y + c[6] * bl.z);
+            py += k * (c[1] * bl.x + c[4] * bl.y + c[7] * bl.z);
+            pz += k * (c[2] * bl.x + c[5] * bl.y + c[8] * bl.z);
+        }
+    }
+    // ---- family 8: conical harmonics (10 lanes: Region-1 conical basis, Region-2 inner)
+    if (lane < CO_NCONIC) {
+        const float4 c = L.conic;
+        const bool g0 = c.w == F(0.0);
+        const F w = g0 ? wc : wi;
+        if (w != F(0.0)) {
+            const int m = (int)c.x;
+            const F X = (g0 ? cxp : xx) + c.z, Y = g0 ? cyp : yy, Z = g0 ? czp : zz;
+            const F ro2 = X * X + Y * Y;
+            const F ro = Dm<F>::sqrt(ro2);
+            const F cf = X / ro, sf = Y / ro;
+            const F r = Dm<F>::sqrt(ro2 + Z * Z);
+            const F cc = Z / r, s = ro / r;
+            const F ch = Dm<F>::sqrt(F(0.5) * (F(1.0) + cc)), sh = Dm<F>::sqrt(F(0.5) * (F(1.0) - cc));
+            const F tnh = sh / ch, cnh = F(1.0) / tnh;
+            // cos/sin(m phi), tnh^m, cnh^m and the (m-1)-th powers
+            F cfm = cf, sfm = sf, tm = tnh, cm = cnh, tm1 = F(1.0), cm1 = F(1.0);
+#pragma unroll
+            for (int j = 2; j <= 5; j++) {
+                if (j <= m) {
+                    const F cn = cfm * cf - sfm * sf;
+                    sfm = sfm * cf + cfm * sf;
+                    cfm = cn;
+                    tm1 = tm;
+                    cm1 = cm;
+                    tm = tm * tnh;
+                    cm = cm * cnh;
+                }
+            }
+            const F fm = (F)m;
+            const F bt = fm * cfm / (r * s) * (tm + cm);
+            const F bf = F(-0.5) * fm * sfm / r * (tm1 / (ch * ch) - cm1 / (sh * sh));
+            const F k = w * c.y;
+            px += k * (bt * cc * cf - bf * sf);
+            py += k * (bt * cc * sf + bf * cf);
+            pz += k * (-bt * s);
+        }
+    }
+    // ---- family 9: Region-2 sheet (15 lanes)
+    if (ws != F(0.0) && lane < CO_NSHEET) {
+        const float *c = L.sheet;
+        const int comp = (int)c[3];
+        F t1, t2, t3;
+        tfac(xks, c[4], c[5], c[6], c[7], &t1, &t2, &t3);
+        const F rh2 = xx * xx + yy * yy;
+        const F r = Dm<F>::sqrt(rh2 + zz * zz);
+        const F rho = Dm<F>::sqrt(rh2);
+        const F c1p = xx / rho, s1p = yy / rho;
+        const F s2p = F(2.0) * s1p * c1p;
+        const F c2p = c1p * c1p - s1p * s1p;
+        const F s3p = s2p * c1p + c2p * s1p;
+        const F c3p = c2p * c1p - s2p * s1p;
+        const F s4p = s3p * c1p + c3p * s1p;
+        const F ct = zz / r;
+        const F ct2 = ct * ct;
+        const F g = c[0] * (comp == 2 ? F(1.0) : ct) * Dm<F>::exp(c[1] * ct2 + c[2]);
+        const bool ycomp = comp == 1;
+        const F m0 = ycomp ? s1p : F(1.0), m1 = ycomp ? s2p : c1p, m2 = ycomp ? s3p : c2p, m3 = ycomp ? s4p : c3p;
+        const F qv = ((m0 * quad(c + 8, t1, t2, t3) + m1 * quad(c + 12, t1, t2, t3)) + m2 * quad(c + 16, t1, t2, t3)) +
+                     m3 * quad(c + 20, t1, t2, t3);
+        const F v = ws * g * qv;
+        if (comp == 0) px += v;
+        else if (comp == 1) py += v;
+        else pz += v;
+    }
+    // ---- Region-2 inner dipole distributions (uniform, counted on lane 0)
+    if (wi != F(0.0) && lane == 0) {
+        const v3<F> d6 = dipdistr(xx - F(.0774), yy, zz, 0);
+        const v3<F> d7 = dipdistr(xx + F(.038), yy, zz, 1);
+        const F k = wi * P.b2ampl;
+        px += k * (F(29.9996) * d6.x + F(262.886) * d7.x);
+        py += k * (F(29.9996) * d6.y + F(262.886) * d7.y);
+        pz += k * (F(29.9996) * d6.z + F(262.886) * d7.z);
+    }
+    S.fx = mk3<F>(px, py, pz);
+    return S;
+}
+
+// Uniform end of lib/t96/t96.c:94-111 once FX is summed.
+__device__ __forceinline__ v3<float> t96_finish(const T96Share &S, v3<float> fx) {
+    typedef float F;
+    const v3<F> q = S.q;
+    if (S.sigma >= F(1.08 + 0.005)) return mk3<F>(-q.x, S.oimfy - q.y, S.oimfz - q.z);
+    if (S.sigma < F(1.08 - 0.005)) return fx;
+    const F fint = F(0.5) * (F(1.0) - (S.sigma - F(1.08)) / F(0.005));
+    const F fext = F(0.5) * (F(1.0) + (S.sigma - F(1.08)) / F(0.005));
+    return mk3<F>((fx.x + q.x) * fint - q.x, (fx.y + q.y) * fint + S.oimfy * fext - q.y,
+                  (fx.z + q.z) * fint + S.oimfz * fext - q.z);
+}
+
+// ---------------------------------------------------------------- IGRF, cooperative
+
+// lib/internal/internal.c:182-275 with one order m per lane (lanes 0..kw-1): each lane
+// advances the sectoral terms to its order, then runs the degree recursion of its
+// column.  Returns this lane's share of (Br, Btheta, Bphi*sin).
+struct IgrfShare {
+    float br, bt, bf;
+    float s, c, cf, sf;
+    bool polar;
+};
+
+__device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G, const float *__restrict__ H,
+                                                     const float *__restrict__ REC, float x, float y, float z,
+                                                     int lane) {
+    typedef float F;
+    IgrfShare S;
+    const F rho2 = x * x + y * y;
+    const F r = Dm<F>::sqrt(rho2 + z * z);
+    const F c = z / r;
+    const F rho = Dm<F>::sqrt(rho2);
+    const F s = rho / r;
+    S.polar = s < F(1.E-5);
+    const F cf = S.polar ? F(1.) : x / rho;
+    const F sf = S.polar ? F(0.) : y / rho;
+    S.s = s;
+    S.c = c;
+    S.cf = cf;
+    S.sf = sf;
+    const F inv_r = F(1.) / r;
+    int nm = 3 + 30 / ((int)r + 2);
+    if (nm > 13) nm = 13;
+    const int kw = nm + 1;
+    const int m = lane + 1;
+    F am = inv_r * inv_r, pmm = F(1.), dmm = F(0.), cm = F(0.), sm = F(1.);
+    for (int j = 1; j < kw; j++) {
+        if (j < m) {
+            const F w = cm;
+            cm = w * cf + sm * sf;
+            sm = sm * cf - w * sf;
+            const F d2 = s * dmm + c * pmm;
+            pmm = s * pmm;
+            dmm = d2;
+            am = am * inv_r;
+        }
+    }
+    F br = F(0.), bt = F(0.), bphi = F(0.);
+    if (m <= kw) {
+        F q = pmm, dq = dmm, q2 = F(0.), dq2 = F(0.), an = am;
+        int idx = m * (m - 1) / 2 + m - 1;
+        F g = G[idx], h = H[idx], rec = REC[idx];
+        for (int n = m; n <= kw; n++) {
+            const F gn = G[idx + n], hn = H[idx + n], recn = REC[idx + n];   // tables padded to 128
+            const F w = g * sm + h * cm;
+            br = br + (an * n) * w * q;
+            bt = bt - an * w * dq;
+            bphi = bphi + an * (g * cm - h * sm) * (S.polar ? dq : q);
+            const F dnext = c * dq - s * q - rec * dq2;
+            const F pnext = c * q - rec * q2;
+            dq2 = dq;
+            q2 = q;
+            dq = dnext;
+            q = pnext;
+            an = an * inv_r;
+            idx += n;
+            g = gn;
+            h = hn;
+            rec = recn;
+        }
+    }
+    S.br = br;
+    S.bt = bt;
+    S.bf = m >= 2 ? bphi * (F)(m - 1) : F(0.);
+    return S;
+}
+
+__device__ __forceinline__ v3<float> igrf_finish(const IgrfShare &S, float br, float bt, float bf) {
+    typedef float F;
+    F bp;
+    if (S.polar) {
+        bp = S.c < F(0.) ? -bf : bf;
+    } else {
+        bp = bf / S.s;
+    }
+    const F he = br * S.s + bt * S.c;
+    return mk3<F>(he * S.cf - bp * S.sf, he * S.sf + bp * S.cf, br * S.c - bt * S.s);
+}
+
+// ---------------------------------------------------------------- total field
+
+// GEO field [nT] of the model set at the GEO position geo_m / GSM position gsm_m [m],
+// identical in every lane of the wavefront.
+template <int INT, int EXT>
+__device__ __forceinline__ v3<float> coop_field(const float *G, const float *H, const float *REC, const float *Af,
+                                                const T96Params<float> &P, const CoopTab &T, v3<double> geo_m,
+                                                v3<double> gsm_m, int lane) {
+    typedef float F;
+    IgrfShare I;
+    T96Share X;
+    F p[6] = {0, 0, 0, 0, 0, 0};
+    if constexpr (INT == INT_IGRF) {
+        I = coop_igrf_share(G, H, REC, (F)(geo_m.x / K_RE), (F)(geo_m.y / K_RE), (F)(geo_m.z / K_RE), lane);
+        p[0] = I.br;
+        p[1] = I.bt;
+        p[2] = I.bf;
+    }
+    if constexpr (EXT == EXT_T96) {
+        X = coop_t96_share(P, T, (F)(gsm_m.x / K_RE), (F)(gsm_m.y / K_RE), (F)(gsm_m.z / K_RE), lane);
+        p[3] = X.fx.x;
+        p[4] = X.fx.y;
+        p[5] = X.fx.z;
+    }
+    wave_sum<6>(p);
+    v3<F> tot = mk3<F>(0, 0, 0);
+    if constexpr (INT == INT_IGRF) tot = igrf_finish(I, p[0], p[1], p[2]);
+    if constexpr (EXT == EXT_T96) {
+        const v3<F> e = t96_finish(X, mk3<F>(p[3], p[4], p[5]));
+        tot.x += Af[0] * e.x + Af[3] * e.y + Af[6] * e.z;
+        tot.y += Af[1] * e.x + Af[4] * e.y + Af[7] * e.z;
+        tot.z += Af[2] * e.x + Af[5] * e.y + Af[8] * e.z;
+    }
+    static_assert(EXT != EXT_TS05, "model not compiled into the cooperative tracer");
+    return tot;
+}
+
+// Copy the lane tables into the workgroup's LDS (all threads of the block).
+__device__ __forceinline__ void stage_coop(const CoopTab *__restrict__ src, CoopTab *dst) {
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+    for (int k = threadIdx.x; k < (int)(sizeof(CoopTab) / 4); k += blockDim.x) d[k] = s[k];
+    __syncthreads();
+}
+
+}  // namespace cor

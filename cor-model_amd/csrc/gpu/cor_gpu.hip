@@ -208,7 +208,22 @@ __global__ void __launch_bounds__(64) penumbra_kernel(const ScanDev *__restrict_
     }
 }
 
-__global__ void reset_head_kernel(unsigned long long *head) { *head = 0ull; }
+__global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop) {
+    *head = 0ull;
+    if (coop) {
+        coop->res = 0ull;
+        coop->head = 0ull;
+        coop->running = 0;
+    }
+}
+
+// wavefronts with at most this many live trajectories hand them to the cooperative mode
+constexpr int COOP_LANES_DEFAULT = 16;
+
+hipError_t build_coop_tables(EpochDev *d_epoch, hipStream_t s) {
+    void *args[] = {&d_epoch};
+    return hipLaunchKernel(coop_build_kernel(), dim3(1), dim3(64), args, 0, s);
+}
 
 }  // namespace
 
@@ -228,6 +243,10 @@ struct cor_trace_plan {
     uint64_t step_limit;
     double tu_angle, cos_tu, mass;
     int grid, block;
+    int coop_lanes;              // 0: per-lane mode only
+    uint32_t run_id;
+    CoopCtl *d_coop;
+    ParkDev *d_park;
     hipEvent_t ev[3];
     bool ran;
 };
@@ -296,6 +315,8 @@ void cor_trace_plan_destroy(cor_trace_plan *p) {
     (void)hipFree(p->out.vc);
     (void)hipFree(p->d_cut);
     (void)hipFree(p->d_stat);
+    (void)hipFree(p->d_coop);
+    (void)hipFree(p->d_park);
     for (int k = 0; k < 3; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     delete p;
@@ -391,8 +412,12 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
         PLAN_TRY(hipMemcpy(p->d_order, order.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice));
     }
 
+    if (p->prec == COR_PREC_MIXED && opts->tail_lanes >= 0)
+        p->coop_lanes = opts->tail_lanes == 0 ? COOP_LANES_DEFAULT : std::min(opts->tail_lanes, 64);
+
     // persistent grid: as many resident blocks as the occupancy allows, but no more
-    // blocks than are needed to give every trajectory a lane at the start
+    // blocks than are needed to give every trajectory a lane at the start (or, with the
+    // cooperative tail, a wavefront to finish in)
     p->block = 256;
     int per_cu = 0;
     PLAN_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p->block, 0));
@@ -402,9 +427,19 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     if (need * 64 < (int64_t)p->block * grid) {
         // small launches: single-wavefront blocks spread over all CUs
         p->block = 64;
-        grid = std::min<int64_t>(need, (int64_t)per_cu * 4 * g_num_cus);
+        grid = std::min<int64_t>(p->coop_lanes > 0 ? total : need, (int64_t)per_cu * 4 * g_num_cus);
     }
     p->grid = (int)std::max<int64_t>(grid, 1);
+    if (p->coop_lanes > 0) {
+        // cooperative tail: lane tables of the epoch, the handover counters and one park
+        // slot per lane of the grid (a lane parks at most one trajectory per run)
+        const size_t slots = (size_t)p->grid * p->block;
+        PLAN_TRY(build_coop_tables(p->d_epoch, g_stream));
+        PLAN_TRY(hipMalloc(&p->d_coop, sizeof(CoopCtl)));
+        PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * slots));
+        PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * slots, g_stream));
+        PLAN_TRY(hipStreamSynchronize(g_stream));
+    }
     for (int k = 0; k < 3; k++) PLAN_TRY(hipEventCreate(&p->ev[k]));
 #undef PLAN_TRY
     *out = p;
@@ -430,7 +465,11 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.cos_tu = p->cos_tu;
     a.mass = p->mass;
     a.out = p->out;
-    reset_head_kernel<<<1, 1, 0, s>>>(p->d_head);
+    a.coop_lanes = p->coop_lanes;
+    a.run_id = ++p->run_id;      // park records of earlier runs never match
+    a.coop = p->d_coop;
+    a.park = p->d_park;
+    reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(p->ev[0], s));
     if (p->n_items > 0) {
@@ -523,13 +562,15 @@ int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const 
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
                   double *geo_nT, int64_t n, int32_t precision) {
     if (!epoch || n < 0 || (n > 0 && (!gsm_m || !geo_nT))) return COR_E_INVALID;
-    if (precision != COR_PREC_FP64 && precision != COR_PREC_MIXED) return COR_E_INVALID;
+    if (precision != COR_PREC_FP64 && precision != COR_PREC_MIXED && precision != COR_PREC_MIXED_WAVE)
+        return COR_E_INVALID;
     int rc = ensure_init();
     if (rc) return rc;
     int intk, ext, cru;
     if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
-    const void *kern = precision == COR_PREC_FP64 ? field_kernel_fp64(intk, ext, cru)
-                                                  : field_kernel_mixed(intk, ext, cru);
+    const void *kern = precision == COR_PREC_FP64    ? field_kernel_fp64(intk, ext, cru)
+                       : precision == COR_PREC_MIXED ? field_kernel_mixed(intk, ext, cru)
+                                                     : field_kernel_coop(intk, ext, cru);
     if (!kern) return COR_E_UNSUPPORTED;
     if (n == 0) return COR_OK;
     EpochDev E;
@@ -541,6 +582,7 @@ int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const 
     if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 3 * n);
     if (e == hipSuccess) e = hipMemcpyAsync(d_e, &E, sizeof(EpochDev), hipMemcpyHostToDevice, g_stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_in, gsm_m, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess && precision == COR_PREC_MIXED_WAVE) e = build_coop_tables(d_e, g_stream);
     if (e == hipSuccess) {
         FieldBatchArgs a;
         a.epoch = d_e;
@@ -548,7 +590,10 @@ int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const 
         a.geo_nT = d_out;
         a.n = n;
         void *args[] = {&a};
-        e = hipLaunchKernel(kern, dim3((unsigned)((n + 255) / 256)), dim3(256), args, 0, g_stream);
+        // one lane per point, or one wavefront per point (4 per block, strided)
+        const int64_t blocks = precision == COR_PREC_MIXED_WAVE ? std::min<int64_t>((n + 3) / 4, 4096)
+                                                                : (n + 255) / 256;
+        e = hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(256), args, 0, g_stream);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(geo_nT, d_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, g_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);

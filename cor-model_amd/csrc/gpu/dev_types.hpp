@@ -33,6 +33,33 @@ constexpr double K_TO_RAD = K_PI / 180.;
 constexpr double K_Q = 1.6021e-19;
 constexpr double K_C = 2.99725e+08;
 
+// Lane tables of the cooperative (one wavefront per trajectory) T96 evaluation: the
+// terms of each family of identical form, one term per lane, with the epoch's
+// amplitudes folded in.  Built on the device once per epoch (build_coop_kernel) and
+// staged in LDS by every workgroup that runs cooperative steps.
+constexpr int CO_NHARM = 47;       // Cartesian harmonics: 3 shield sets x 9, birk1shld 16, birk2shl 4
+constexpr int CO_NICON = 3;        // interconnection harmonics, one y harmonic per lane
+constexpr int CO_NCONE = 6;        // conical sheets: ring current 2, tail disk 4
+constexpr int CO_NCYL = 6;         // cylindrical harmonics
+constexpr int CO_NT87 = 3;         // tail87: central sheet and its two images
+constexpr int CO_NDIP_LOOP = 21;   // Region-1 loop-basis dipoles
+constexpr int CO_NDIP = 67;        // ... followed by 46 conical-basis dipoles
+constexpr int CO_NCIRC = 18;       // current loops: Region-1 loop basis 3, Region-2 outer 11, inner 4
+constexpr int CO_NCONIC = 10;      // conical harmonics: Region-1 5, Region-2 inner 5
+constexpr int CO_NSHEET = 15;      // Region-2 sheet: 3 components x 5 latitude terms
+
+struct alignas(16) CoopTab {   // rows are read as float4
+    float harm[64][4];           // 1/p, 1/r, sqrt(1/p^2 + 1/r^2), coefficient x amplitude
+    float cone[8][4];            // f x amplitude, beta, x offset, dd slope (0 ring current, 10 disk)
+    float cyl[8][4];             // a x xappa^3, scale, kind (0: j1 terms, 1: mixed terms), 0
+    float t87[4][4];             // z offset, weight x amplitude, 0, 0
+    float dip[68][8];            // position (3), 0, moment x amplitude (3), 0
+    float circ[24][16];          // rotation rows (9), centre (3), radius, coefficient x amplitude, group, 0
+    float conic[16][4];          // order, coefficient x amplitude, x offset, group
+    float sheet[16][24];         // alpha, beta, gamma, component, d1, d2, d3, d3^4, 16 coefficients
+    float icon[4][24];           // per y harmonic: the k = 0 term and the series of the k = 1, 2 pair
+};
+
 struct EpochDev {
     double G[NCOEF], H[NCOEF], REC[NCOEF];
     double A[9];                 // GEO->GSM rows A1, A2, A3
@@ -45,6 +72,26 @@ struct EpochDev {
     int32_t pad;
     double shG[SH_NM_MAX * SH_NM_MAX], shH[SH_NM_MAX * SH_NM_MAX];
     float shGf[SH_NM_MAX * SH_NM_MAX], shHf[SH_NM_MAX * SH_NM_MAX];
+    CoopTab coop;
+};
+
+// State of a trajectory handed from the per-lane mode to the cooperative mode, taken
+// at the top of a step (the field is re-evaluated from the position).
+struct ParkDev {
+    double pos[3], vel[3];
+    double v, time, alength, zn;
+    int64_t gid;
+    uint32_t step, attempts;
+    int32_t nk1;
+    uint32_t ready;              // = run id once the record is complete
+};
+
+// Work-handover counters of one run (cleared by the reset kernel).
+struct CoopCtl {
+    unsigned long long res;      // park slots reserved
+    unsigned long long head;     // tickets taken by cooperative workers
+    int32_t running;             // wavefronts still in the per-lane mode
+    int32_t pad;
 };
 
 struct ScanDev {
@@ -76,6 +123,13 @@ struct TraceArgs {
     double cos_tu;
     double mass;
     TraceOut out;
+    // cooperative tail (mixed precision): once the queue is drained, a wavefront with
+    // at most coop_lanes live trajectories parks them and turns into a worker that runs
+    // parked trajectories one per wavefront (0: per-lane mode only)
+    int32_t coop_lanes;
+    uint32_t run_id;
+    CoopCtl *coop;
+    ParkDev *park;
 };
 
 }  // namespace cor

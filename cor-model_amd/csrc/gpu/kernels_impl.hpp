@@ -28,6 +28,28 @@ __global__ void __launch_bounds__(256) field_batch_kernel(FieldBatchArgs a) {
     a.geo_nT[3 * i + 2] = out.z;
 }
 
+// The same batch with the cooperative evaluator: one wavefront per point, points
+// strided over the grid's wavefronts.
+template <int INT, int EXT, int CRU>
+__global__ void __launch_bounds__(256) field_coop_kernel(FieldBatchArgs a) {
+    __shared__ float s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
+    __shared__ CoopTab s_coop;
+    const EpochView<float> E = stage_epoch<float>(a.epoch, s_g, s_h, s_rec);
+    stage_coop(&a.epoch->coop, &s_coop);
+    const int lane = __lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; i < a.n; i += waves) {
+        const v3<double> gsm = mk3(a.gsm_m[3 * i], a.gsm_m[3 * i + 1], a.gsm_m[3 * i + 2]);
+        const v3<double> geo = rot_gsm_to_geo(E.A, gsm);
+        const v3<float> f = coop_field<INT, EXT>(E.G, E.H, E.REC, E.Af, E.t96, s_coop, geo, gsm, lane);
+        if (lane == 0) {
+            a.geo_nT[3 * i] = f.x;
+            a.geo_nT[3 * i + 1] = f.y;
+            a.geo_nT[3 * i + 2] = f.z;
+        }
+    }
+}
+
 #define COR_MODEL_SETS(X)                  \
     X(INT_IGRF, EXT_NONE, CRU_NONE)        \
     X(INT_IGRF, EXT_T96, CRU_NONE)         \
@@ -46,6 +68,15 @@ template <int PREC>
 const void *field_kernel_table(int intk, int ext, int cru) {
 #define COR_PICK(I, E, C) \
     if (intk == I && ext == E && cru == C) return (const void *)&field_batch_kernel<PREC, I, E, C>;
+    COR_MODEL_SETS(COR_PICK)
+#undef COR_PICK
+    return nullptr;
+}
+
+template <int PREC>
+const void *field_coop_table(int intk, int ext, int cru) {
+#define COR_PICK(I, E, C) \
+    if (intk == I && ext == E && cru == C) return (const void *)&field_coop_kernel<I, E, C>;
     COR_MODEL_SETS(COR_PICK)
 #undef COR_PICK
     return nullptr;

@@ -9,4 +9,12 @@ const void *trace_kernel_mixed(int intk, int ext, int cru) {
 const void *field_kernel_mixed(int intk, int ext, int cru) {
     return field_kernel_table<PREC_MIXED>(intk, ext, cru);
 }
+const void *field_kernel_coop(int intk, int ext, int cru) {
+    return field_coop_table<PREC_MIXED>(intk, ext, cru);
+}
+
+__global__ void build_coop_kernel(EpochDev *E) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) build_coop_tab(E->t96d, E->coop);
+}
+const void *coop_build_kernel() { return (const void *)&build_coop_kernel; }
 }  // namespace cor

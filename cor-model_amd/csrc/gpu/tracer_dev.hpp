@@ -22,6 +22,7 @@
 #include "dev_types.hpp"
 #include "igrf_dev.hpp"
 #include "t96_dev.hpp"
+#include "coop_dev.hpp"
 
 namespace cor {
 
@@ -362,7 +363,8 @@ template <class EV>
 __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v3<float> fld,
                                               v3<double> &pos, v3<double> &vel, double &v, double b,
                                               double zn, int &nk0, int nk1, unsigned int &step,
-                                              uint32_t &attempts, double &time, double &alength, int64_t gid) {
+                                              uint32_t &attempts, double &time, double &alength, int64_t gid,
+                                              bool writer = true) {
     constexpr double inv_c = 1.0 / K_C, inv_re = 1.0 / K_RE, inv_re2 = 1.0 / (K_RE * K_RE);
     constexpr double re2 = K_RE * K_RE, r25sq = (25. * K_RE) * (25. * K_RE);
     const double inv_mass = 1.0 / a.mass;
@@ -404,7 +406,7 @@ __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v
         if (rr2 >= r25sq || paus > 1.) {
             const double r = sqrt(rr2) * inv_re;
             if ((r >= 25. && fabs(r - 25.) < 0.002) || (r <= 25. && paus > 1. && fabs(paus - 1.) < 0.002)) {
-                write_exit(a, gid, pos, vel, v, r, time, alength);
+                if (writer) write_exit(a, gid, pos, vel, v, r, time, alength);
                 return ST_ALLOWED;
             }
             nk0 = (int)((unsigned)nk0 * 2u);
@@ -430,6 +432,102 @@ struct FieldT<PREC_FP64> { typedef double T; };
 template <>
 struct FieldT<PREC_MIXED> { typedef float T; };
 
+// ---------------------------------------------------------------- cooperative tail
+
+__device__ __forceinline__ unsigned long long load_acquire(unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int load_acquire(int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Hand this wavefront's live trajectories to the cooperative workers (called with the
+// whole wavefront active, at the top of a step): lane 0 reserves one park slot per live
+// lane, each live lane writes its state and publishes the slot with the run id.
+__device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool active, int64_t gid,
+                                           v3<double> pos, v3<double> vel, double v, double time,
+                                           double alength, double zn, unsigned int step, uint32_t attempts,
+                                           int nk1) {
+    const uint64_t live = __ballot(active);
+    if (!live) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.coop->res, (unsigned long long)__popcll(live));
+    base = __shfl(base, 0);
+    if (active) {
+        ParkDev *P = a.park + base + __popcll(live & ((1ull << lane) - 1ull));
+        P->pos[0] = pos.x;
+        P->pos[1] = pos.y;
+        P->pos[2] = pos.z;
+        P->vel[0] = vel.x;
+        P->vel[1] = vel.y;
+        P->vel[2] = vel.z;
+        P->v = v;
+        P->time = time;
+        P->alength = alength;
+        P->zn = zn;
+        P->gid = gid;
+        P->step = step;
+        P->attempts = attempts;
+        P->nk1 = nk1;
+        __hip_atomic_store(&P->ready, a.run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Cooperative worker: takes tickets for park slots in order and runs each parked
+// trajectory to its end with the whole wavefront.  A ticket beyond the reserved slots
+// waits until a slot appears or no wavefront is left in the per-lane mode (then the
+// reserved count is final and the worker retires).  Never waits for another worker,
+// and parkers never wait at all, so every wait ends.
+template <int INT, int EXT>
+__device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const CoopTab &T, int lane) {
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(&a.coop->head, 1ull);
+        t = __shfl(t, 0);
+        bool got;
+        for (;;) {
+            if (t < load_acquire(&a.coop->res)) {
+                got = true;
+                break;
+            }
+            if (load_acquire(&a.coop->running) == 0) {
+                got = t < load_acquire(&a.coop->res);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(32);
+        }
+        if (!got) return;
+        ParkDev *P = a.park + t;
+        while (__hip_atomic_load(&P->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.run_id)
+            __builtin_amdgcn_s_sleep(2);
+        v3<double> pos = mk3(P->pos[0], P->pos[1], P->pos[2]);
+        v3<double> vel = mk3(P->vel[0], P->vel[1], P->vel[2]);
+        double v = P->v, time = P->time, alength = P->alength;
+        const double zn = P->zn;
+        const int64_t gid = P->gid;
+        unsigned int step = P->step;
+        uint32_t attempts = P->attempts;
+        const int nk1 = P->nk1;
+        int nk0 = nk1;
+        for (;;) {
+            const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
+            const v3<float> f = coop_field<INT, EXT>(E.G, E.H, E.REC, E.Af, E.t96, T, pos, gsm, lane);
+            const double fx = f.x, fy = f.y, fz = f.z;
+            const double b = 1.E-09 * sqrt(fx * fx + fy * fy + fz * fz);
+            const int status = attempts_mixed(a, E, f, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength,
+                                              gid, lane == 0);
+            if (status >= 0) {
+                if (lane == 0) {
+                    a.out.status[gid] = status;
+                    a.out.steps[gid] = step;
+                    a.out.attempts[gid] = attempts;
+                }
+                break;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- the kernel
 
 // Register budget: COR_WAVES_PER_EU wavefronts per SIMD (set per translation unit; the
@@ -445,6 +543,15 @@ trace_kernel(TraceArgs a) {
     __shared__ FT s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
     const EpochView<FT> E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
     const int lane = __lane_id();
+    const CoopTab *ctab = nullptr;
+    if constexpr (PREC == PREC_MIXED) {
+        __shared__ CoopTab s_coop;
+        if (a.coop_lanes > 0) {
+            stage_coop(&a.epoch->coop, &s_coop);
+            if (lane == 0) atomicAdd(&a.coop->running, 1);
+        }
+        ctab = &s_coop;
+    }
 
     bool active = false;
     bool queue_open = true;
@@ -456,6 +563,9 @@ trace_kernel(TraceArgs a) {
     int nk0 = 0, nk1 = 0;
     unsigned int step = 0;
     uint32_t attempts = 0;
+    // queue head as read one step earlier: a wavefront whose lanes are all live never
+    // refills, so it learns that other wavefronts drained the queue from this
+    unsigned long long head_seen = 0;
 
     for (;;) {
         // ---- 1. refill ended lanes from the queue
@@ -493,6 +603,18 @@ trace_kernel(TraceArgs a) {
                 }
             }
         }
+        if constexpr (PREC == PREC_MIXED) {
+            // queue drained: a sparse wavefront hands its trajectories to the
+            // cooperative mode and becomes a worker itself
+            if (a.coop_lanes > 0) {
+                if (queue_open && head_seen >= (unsigned long long)a.n_items) queue_open = false;
+                if (!queue_open && __popcll(__ballot(active)) <= a.coop_lanes) {
+                    park_lanes(a, lane, active, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
+                    break;
+                }
+                if (queue_open) head_seen = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
         if (!__any(active)) break;
 
         if (active) {
@@ -524,6 +646,12 @@ trace_kernel(TraceArgs a) {
                 a.out.attempts[gid] = attempts;
                 active = false;
             }
+        }
+    }
+    if constexpr (PREC == PREC_MIXED) {
+        if (a.coop_lanes > 0) {
+            if (lane == 0) __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            coop_worker<INT, EXT>(a, E, *ctab, lane);
         }
     }
 }

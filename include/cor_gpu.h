@@ -33,7 +33,9 @@ enum {
     COR_E_NOMEM = -5
 };
 
-enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1 };
+/* COR_PREC_MIXED_WAVE: cor_gpu_field only -- the mixed-precision model evaluated one
+ * wavefront per point, as the tracer's cooperative mode evaluates it. */
+enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1, COR_PREC_MIXED_WAVE = 2 };
 enum { COR_INTERNAL_NONE = 0, COR_INTERNAL_IGRF = 1 };
 enum { COR_EXTERNAL_NONE = 0, COR_EXTERNAL_T96 = 1, COR_EXTERNAL_TS05 = 2 };
 enum { COR_CRUSTAL_NONE = 0, COR_CRUSTAL_SH = 1 };
@@ -72,6 +74,11 @@ typedef struct {
     double particle_mass;     /* DEF_HM0_P or DEF_HM0_E */
     int32_t precision;        /* COR_PREC_FP64 or COR_PREC_MIXED */
     int32_t queue_order;      /* 0 = rigidity order, 1 = longest-expected first */
+    int32_t tail_lanes;       /* COR_PREC_MIXED only: once the queue is drained, a wavefront
+                               * with at most this many live trajectories hands them to the
+                               * cooperative mode (one wavefront per trajectory); 0 = the
+                               * library default, negative = per-lane mode only */
+    int32_t pad;
 } cor_trace_opts;
 
 /* Per trajectory.  The doubles are set only for allowed trajectories: exit v/c, exit

@@ -5,7 +5,8 @@ The reference ships no tests or fixtures of its own, so these vectors are the pi
 inputs and the reference's outputs, nothing else.  Run in the container that has the
 reference sources (make -C oracle ref first):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          (everything)
+    python tests/golden/make_golden.py ts05     (the TS05 fixtures only)
 """
 import sys
 from pathlib import Path
@@ -52,7 +53,33 @@ def t96_points(rng, n):
     return np.vstack([p] + lines)
 
 
+TS05_PARAMS = [[2, -20, 0, -2, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5],
+               [8, -250, -4, 6, 1.5, 0.2, 1.9, 0.7, 1.1, 0.3], [0.6, 10, 1, 1, 0, 0, 0, 0, 0, 0]]
+
+
+def ts05_fixtures():
+    """TS05 field vectors and one TS05 cutoff run (lib/ts05/ts05.c, -e ts05)."""
+    ref = L.RefLib()
+    rng = np.random.default_rng(20240505)
+    out = {"ts05_pts": t96_points(rng, 3000)}
+    for k, pm in enumerate(TS05_PARAMS):
+        out[f"ts05_params{k}"] = np.array(pm, dtype=float)
+        out[f"ts05_b{k}"] = ref.ts05(pm, out["ts05_pts"])
+    np.savez_compressed(HERE / "ts05_vectors.npz", **out)
+    name = "ts05_storm_nk100"
+    inp = HERE / f"{name}.in"
+    # file order: Dst Pdyn By Bz, then W1..W6
+    L.write_cutoff_input(inp, 1.0, -1.0, 20.0, 1.00280, 52.0, 30.0, 52.0, 30.0, nk1=100, dl=0.25,
+                         ext_params=[-100, 3, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5])
+    txt = L.run_ref_cutoff(inp, internal="igrf14", external="ts05", steps=20000)
+    (HERE / f"{name}.out").write_text(txt)
+    print(name, txt.strip().splitlines()[-1])
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "ts05":
+        ts05_fixtures()
+        return
     ref = L.RefLib()
     rng = np.random.default_rng(20240501)
     out = {}
@@ -92,6 +119,7 @@ def main():
         txt = L.run_ref_cutoff(inp, internal="igrf14", external=r["external"], steps=r.get("steps", 1000000))
         (HERE / f"{name}.out").write_text(txt)
         print(name, txt.strip().splitlines()[-1])
+    ts05_fixtures()
 
 
 if __name__ == "__main__":

@@ -4,7 +4,8 @@ FP64 path: the same expression order as the reference, compiled without FMA
 contraction; the remaining difference is OCML vs glibc transcendentals (a few ULP),
 so the bar is 1e-12 relative to |B|.
 
-Mixed path: FP32 field.  T96's shielding fits cancel terms 1e4-1e5 times larger than
+Mixed paths (one point per lane, and one point per wavefront as the tracer's
+cooperative mode evaluates it): FP32 field.  T96's shielding fits cancel terms 1e4-1e5 times larger than
 their sum, so its FP32 error is measured against the field the particle sees: |dB|
 divided by |B| + B_dip(r) (B_dip = 30574 nT / r^3, the dipole scale, which is what
 IGRF adds on top of the external field).  Bar: max 1e-3, 99th percentile 1e-4.
@@ -74,6 +75,26 @@ def test_field_mixed_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
     err = np.linalg.norm(got - want, axis=1) / scale
     print(name, "mixed max rel err", err.max(), "p99", np.quantile(err, 0.99),
           "max abs nT", np.linalg.norm(got - want, axis=1).max())
+    assert np.all(np.isfinite(got))
+    assert err.max() < 1e-3
+    assert np.quantile(err, 0.99) < 1e-4
+
+
+@pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
+def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
+    """The tracer's cooperative evaluator (one wavefront per point, one term family per
+    SIMD pass, one cross-lane sum): same FP32 bar as the per-lane mixed path."""
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    if parmod:
+        for k, v in enumerate(parmod):
+            po.PARMOD[k] = v
+    got = cor.field(cor.model_set(**kinds), p, pts, cor.PREC_MIXED_WAVE)
+    want = oracle.field(L.models(**kinds), po, pts)
+    r = np.linalg.norm(pts, axis=1) / RE
+    scale = np.linalg.norm(want, axis=1) + 30574.0 / r ** 3
+    err = np.linalg.norm(got - want, axis=1) / scale
+    print(name, "wave max rel err", err.max(), "p99", np.quantile(err, 0.99))
     assert np.all(np.isfinite(got))
     assert err.max() < 1e-3
     assert np.quantile(err, 0.99) < 1e-4

@@ -2,8 +2,8 @@
 tests/golden/make_golden.py from the reference compiled from its sources).
 
 Same compiler flags and the same libm: the restatement is expected to be bit-identical
-to the reference -- IGRF-14 epoch setup, IGRF synthesis, T96, and the complete text of
-cutoff runs (every allowed-trajectory line and the Rl/Ru/Rc summary).
+to the reference -- IGRF-14 epoch setup, IGRF synthesis, T96, TS05, and the complete
+text of cutoff runs (every allowed-trajectory line and the Rl/Ru/Rc summary).
 """
 from pathlib import Path
 
@@ -40,6 +40,16 @@ def test_igrf_bitwise(oracle, vec):
 
 
 @pytest.mark.parametrize("k", range(4))
+def test_ts05_bitwise(oracle, k):
+    """lib/ts05/*.c at four driver sets (quiet to storm, W1..W6 on and off), bit for bit."""
+    tv = np.load(GOLD / "ts05_vectors.npz")
+    got = oracle.ts05(tv[f"ts05_params{k}"], tv["ts05_pts"])
+    want = tv[f"ts05_b{k}"]
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), np.argwhere(~same)[:5]
+
+
+@pytest.mark.parametrize("k", range(4))
 def test_t96_bitwise(oracle, vec, k):
     got = oracle.t96(vec[f"t96_params{k}"], vec["t96_pts"])
     want = vec[f"t96_b{k}"]
@@ -47,7 +57,7 @@ def test_t96_bitwise(oracle, vec, k):
     assert same.all(), np.argwhere(~same)[:5]
 
 
-def _cutoff_run(oracle, name, models, parmod):
+def _cutoff_run(oracle, name, models, parmod, limit):
     txt = (GOLD / f"{name}.out").read_text()
     ref_lines = [ln for ln in txt.splitlines() if len(ln.split()) == 9 and _numeric(ln)]
     _, ref_cut = L.parse_cutoff_output(txt)
@@ -57,7 +67,6 @@ def _cutoff_run(oracle, name, models, parmod):
     _, p = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
     for k, v in enumerate(parmod):
         p.PARMOD[k] = v
-    limit = 20000 if "limit" in name else 1000000
     sc = L.OScan(r0, the0, fi0, the1, fi1, rig, dl, rk, zn, nk1, 0)
     tr, cut = oracle.trace_scan(sc, models, p, limit)
     lines = []
@@ -78,13 +87,14 @@ def _numeric(ln):
         return False
 
 
-@pytest.mark.parametrize("name,models,parmod", [
-    ("config1_igrf_nk100", dict(internal=1, external=0), []),
-    ("t96_equator_nk400", dict(internal=1, external=1), [2, -20, 0, -2]),
-    ("t96_mid_nk400", dict(internal=1, external=1), [2, -20, 0, -2]),
-    ("t96_storm_limit_nk100", dict(internal=1, external=1), [6, -150, 4, -8]),
+@pytest.mark.parametrize("name,models,parmod,limit", [
+    ("config1_igrf_nk100", dict(internal=1, external=0), [], 1000000),
+    ("t96_equator_nk400", dict(internal=1, external=1), [2, -20, 0, -2], 1000000),
+    ("t96_mid_nk400", dict(internal=1, external=1), [2, -20, 0, -2], 1000000),
+    ("t96_storm_limit_nk100", dict(internal=1, external=1), [6, -150, 4, -8], 20000),
+    ("ts05_storm_nk100", dict(internal=1, external=2), [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], 20000),
 ])
-def test_cutoff_output_identical(oracle, name, models, parmod):
-    lines, ref_lines, summary, txt = _cutoff_run(oracle, name, L.models(**models), parmod)
+def test_cutoff_output_identical(oracle, name, models, parmod, limit):
+    lines, ref_lines, summary, txt = _cutoff_run(oracle, name, L.models(**models), parmod, limit)
     assert lines == ref_lines
     assert summary in txt
