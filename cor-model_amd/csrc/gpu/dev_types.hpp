@@ -13,6 +13,7 @@
 #pragma once
 #include <stdint.h>
 #include "t96_dev.hpp"
+#include "ts05_dev.hpp"
 
 namespace cor {
 
@@ -67,6 +68,8 @@ struct EpochDev {
     float Af[9];
     T96Params<double> t96d;
     T96Params<float> t96f;
+    TS05Params<double> ts05d;
+    TS05Params<float> ts05f;
     double parmod[10];
     int32_t sh_nm;               // SH table size (0 = none)
     int32_t pad;

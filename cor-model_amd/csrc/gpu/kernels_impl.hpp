@@ -50,10 +50,16 @@ __global__ void __launch_bounds__(256) field_coop_kernel(FieldBatchArgs a) {
     }
 }
 
-#define COR_MODEL_SETS(X)                  \
+#define COR_COOP_MODEL_SETS(X)             \
     X(INT_IGRF, EXT_NONE, CRU_NONE)        \
     X(INT_IGRF, EXT_T96, CRU_NONE)         \
     X(INT_NONE, EXT_T96, CRU_NONE)
+#define COR_MODEL_SETS(X)                  \
+    COR_COOP_MODEL_SETS(X)                 \
+    X(INT_IGRF, EXT_TS05, CRU_NONE)        \
+    X(INT_NONE, EXT_TS05, CRU_NONE)        \
+    X(INT_NONE, EXT_NONE, CRU_SH)          \
+    X(INT_NONE, EXT_TS05, CRU_SH)
 
 template <int PREC>
 const void *trace_kernel_table(int intk, int ext, int cru) {
@@ -77,7 +83,7 @@ template <int PREC>
 const void *field_coop_table(int intk, int ext, int cru) {
 #define COR_PICK(I, E, C) \
     if (intk == I && ext == E && cru == C) return (const void *)&field_coop_kernel<I, E, C>;
-    COR_MODEL_SETS(COR_PICK)
+    COR_COOP_MODEL_SETS(COR_PICK)
 #undef COR_PICK
     return nullptr;
 }

@@ -23,6 +23,7 @@
 #include "igrf_dev.hpp"
 #include "t96_dev.hpp"
 #include "coop_dev.hpp"
+#include "sh_dev.hpp"
 
 namespace cor {
 
@@ -37,7 +38,15 @@ struct EpochView {
     double A[9];
     float Af[9];
     T96Params<FT> t96;
+    const TS05Params<FT> *ts05;          // in HBM (read with scalar loads)
+    const TS05Params<double> *ts05d;     // FP64 constants of the shield sums
+    const FT *shG, *shH;                 // SH model tables in HBM (uniform indices)
+    int sh_nm;
 };
+__device__ __forceinline__ const double *sh_tab(const EpochDev *E, int k, double) { return k ? E->shH : E->shG; }
+__device__ __forceinline__ const float *sh_tab(const EpochDev *E, int k, float) { return k ? E->shHf : E->shGf; }
+__device__ __forceinline__ const TS05Params<double> *ts05_of(const EpochDev *E, double) { return &E->ts05d; }
+__device__ __forceinline__ const TS05Params<float> *ts05_of(const EpochDev *E, float) { return &E->ts05f; }
 
 __device__ __forceinline__ const T96Params<double> &t96_of(const EpochDev *E, double) { return E->t96d; }
 __device__ __forceinline__ const T96Params<float> &t96_of(const EpochDev *E, float) { return E->t96f; }
@@ -69,6 +78,11 @@ __device__ __forceinline__ EpochView<FT> stage_epoch(const EpochDev *__restrict_
         V.Af[k] = E->Af[k];
     }
     V.t96 = t96_of(E, FT(0));
+    V.ts05 = ts05_of(E, FT(0));
+    V.ts05d = &E->ts05d;
+    V.shG = sh_tab(E, 0, FT(0));
+    V.shH = sh_tab(E, 1, FT(0));
+    V.sh_nm = E->sh_nm;
     return V;
 }
 
@@ -108,7 +122,21 @@ __device__ __forceinline__ v3<double> field_fp64(const EpochView<double> &E, v3<
         tot.y += bs.y;
         tot.z += bs.z;
     }
-    static_assert(EXT != EXT_TS05 && CRU == CRU_NONE, "model not compiled into this tracer");
+    if (EXT == EXT_TS05) {
+        const v3<double> bs = ts05_field<double>(*E.ts05, *E.ts05d, g.x, g.y, g.z);
+        tot.x += bs.x;
+        tot.y += bs.y;
+        tot.z += bs.z;
+    }
+    if (CRU == CRU_SH) {
+        // src/models.c:179-192: GSM -> GEO, synthesis, GEO -> GSM, accumulate
+        const v3<double> geo = rot_gsm_to_geo(E.A, g);
+        const v3<double> bg = sh_geo<double, SH_NM_MAX>(E.shG, E.shH, E.sh_nm, geo.x, geo.y, geo.z);
+        const v3<double> bs = rot_geo_to_gsm(E.A, bg);
+        tot.x += bs.x;
+        tot.y += bs.y;
+        tot.z += bs.z;
+    }
     return rot_gsm_to_geo(E.A, tot);
 }
 
@@ -129,7 +157,20 @@ __device__ __forceinline__ v3<float> field_mixed(const EpochView<float> &E, v3<d
         tot.y += bg.y;
         tot.z += bg.z;
     }
-    static_assert(EXT != EXT_TS05 && CRU == CRU_NONE, "model not compiled into this tracer");
+    if (EXT == EXT_TS05) {
+        const float x = (float)(gsm_m.x / K_RE), y = (float)(gsm_m.y / K_RE), z = (float)(gsm_m.z / K_RE);
+        const v3<float> bg = rot_gsm_to_geo_f(E.Af, ts05_field<float>(*E.ts05, *E.ts05d, x, y, z));
+        tot.x += bg.x;
+        tot.y += bg.y;
+        tot.z += bg.z;
+    }
+    if (CRU == CRU_SH) {
+        const float x = (float)(geo_m.x / K_RE), y = (float)(geo_m.y / K_RE), z = (float)(geo_m.z / K_RE);
+        const v3<float> bg = sh_geo<float, SH_NM_MAX>(E.shG, E.shH, E.sh_nm, x, y, z);
+        tot.x += bg.x;
+        tot.y += bg.y;
+        tot.z += bg.z;
+    }
     return tot;
 }
 
@@ -544,7 +585,8 @@ trace_kernel(TraceArgs a) {
     const EpochView<FT> E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
     const int lane = __lane_id();
     const CoopTab *ctab = nullptr;
-    if constexpr (PREC == PREC_MIXED) {
+    constexpr bool kCoop = PREC == PREC_MIXED && EXT != EXT_TS05 && CRU == CRU_NONE;   // the cooperative evaluator's models
+    if constexpr (kCoop) {
         __shared__ CoopTab s_coop;
         if (a.coop_lanes > 0) {
             stage_coop(&a.epoch->coop, &s_coop);
@@ -566,6 +608,9 @@ trace_kernel(TraceArgs a) {
     // queue head as read one step earlier: a wavefront whose lanes are all live never
     // refills, so it learns that other wavefronts drained the queue from this
     unsigned long long head_seen = 0;
+    // cooperative workers waiting for work (tickets taken minus slots reserved), as
+    // read one step earlier
+    long long idle_seen = 0;
 
     for (;;) {
         // ---- 1. refill ended lanes from the queue
@@ -603,16 +648,25 @@ trace_kernel(TraceArgs a) {
                 }
             }
         }
-        if constexpr (PREC == PREC_MIXED) {
+        if constexpr (kCoop) {
             // queue drained: a sparse wavefront hands its trajectories to the
             // cooperative mode and becomes a worker itself
+            // and parks when its live trajectories are few, or when enough workers are
+            // idle to take every one of them (a worker's step is ~5x shorter than a
+            // per-lane step, so idle workers are always the faster way)
             if (a.coop_lanes > 0) {
                 if (queue_open && head_seen >= (unsigned long long)a.n_items) queue_open = false;
-                if (!queue_open && __popcll(__ballot(active)) <= a.coop_lanes) {
-                    park_lanes(a, lane, active, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
-                    break;
+                if (!queue_open) {
+                    const int live = __popcll(__ballot(active));
+                    if (live <= a.coop_lanes || idle_seen >= (long long)live) {
+                        park_lanes(a, lane, active, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
+                        break;
+                    }
+                    idle_seen = (long long)(__hip_atomic_load(&a.coop->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                                            __hip_atomic_load(&a.coop->res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                } else {
+                    head_seen = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                if (queue_open) head_seen = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         if (!__any(active)) break;
@@ -648,7 +702,7 @@ trace_kernel(TraceArgs a) {
             }
         }
     }
-    if constexpr (PREC == PREC_MIXED) {
+    if constexpr (kCoop) {
         if (a.coop_lanes > 0) {
             if (lane == 0) __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             coop_worker<INT, EXT>(a, E, *ctab, lane);

@@ -77,7 +77,8 @@ typedef struct {
     int32_t tail_lanes;       /* COR_PREC_MIXED only: once the queue is drained, a wavefront
                                * with at most this many live trajectories hands them to the
                                * cooperative mode (one wavefront per trajectory); 0 = the
-                               * library default, negative = per-lane mode only */
+                               * library default (4), negative = per-lane mode only; fuller
+                               * wavefronts also hand over when enough workers are idle */
     int32_t pad;
 } cor_trace_opts;
 

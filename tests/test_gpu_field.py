@@ -98,3 +98,79 @@ def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
     assert np.all(np.isfinite(got))
     assert err.max() < 1e-3
     assert np.quantile(err, 0.99) < 1e-4
+
+
+TS05_CASES = [
+    ("ts05", dict(internal=0, external=2), [2, -20, 0, -2, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], box_points(6, 4000)),
+    ("igrf+ts05 storm", dict(internal=1, external=2), [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5],
+     box_points(7, 4000)),
+    ("igrf+ts05 drivers", dict(internal=1, external=2), [8, -250, -4, 6, 1.5, 0.2, 1.9, 0.7, 1.1, 0.3],
+     box_points(8, 3000)),
+]
+
+
+@pytest.mark.parametrize("name,kinds,parmod,pts", TS05_CASES, ids=[c[0] for c in TS05_CASES])
+def test_field_ts05(cor, oracle, name, kinds, parmod, pts):
+    """TS05 (lib/ts05/*.c).  FP64: the reference's central differences (steps 1e-6 and
+    1e-4) divide the few-ulp OCML/glibc differences of the differenced potentials by
+    the step, ~1e-11 of |B| typically (p99 bar 1e-9); where TS05 alone is weak against
+    the potentials it differences the relative figure grows, so the worst point's bar
+    is 1e-7 of |B|.  Mixed: the T96 bar; its derivatives are exact (dual numbers) and
+    its cancelling harmonic sums FP64."""
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    for k, v in enumerate(parmod):
+        po.PARMOD[k] = v
+    want = oracle.field(L.models(**kinds), po, pts)
+    got = cor.field(cor.model_set(**kinds), p, pts, cor.PREC_FP64)
+    err = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    print(name, "fp64 max rel err", err.max(), "p99", np.quantile(err, 0.99), "at |B|",
+          np.linalg.norm(want[np.argmax(err)]))
+    assert np.all(np.isfinite(got)) and err.max() < 1e-7 and np.quantile(err, 0.99) < 1e-9
+    got = cor.field(cor.model_set(**kinds), p, pts, cor.PREC_MIXED)
+    r = np.linalg.norm(pts, axis=1) / RE
+    scale = np.linalg.norm(want, axis=1) + 30574.0 / r ** 3
+    err = np.linalg.norm(got - want, axis=1) / scale
+    print(name, "mixed max rel err", err.max(), "p99", np.quantile(err, 0.99))
+    assert np.all(np.isfinite(got)) and err.max() < 1e-3 and np.quantile(err, 0.99) < 1e-4
+
+
+def seeded_sh(seed, nm=21):
+    """A degree-20 table (nm = 21 rows) with an IGRF-like spectrum, row n, column m, nT."""
+    rng = np.random.default_rng(seed)
+    g = np.zeros((nm, nm))
+    h = np.zeros((nm, nm))
+    for n in range(1, nm):
+        sc = 30000.0 * 0.3 ** n
+        for m in range(n + 1):
+            g[n, m] = sc * rng.normal()
+            h[n, m] = sc * rng.normal() if m else 0.0
+    return g, h
+
+
+@pytest.mark.parametrize("name,ext,parmod", [
+    ("sh", 0, None),
+    ("sh+ts05", 2, [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5]),
+])
+def test_field_sh(cor, oracle, name, ext, parmod):
+    """Generic SH synthesis (lib/chaos/chaos.c:118-227) to degree 20, alone and with
+    TS05 as in the 1-degree map configuration: FP64 within 1e-12 of |B| (1e-8 with
+    TS05's differences), mixed at the FP32 bar."""
+    g, h = seeded_sh(11)
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    for k, v in enumerate(parmod or []):
+        po.PARMOD[k] = v
+    pts = points(9, 3000, rmin=1.0, rmax=12.0)
+    ms = cor.model_set(cor.INTERNAL_NONE, ext, sh=(g, h))
+    want = oracle.field(L.models(internal=0, external=ext, crustal=1, chaos=(g, h)), po, pts)
+    got = cor.field(ms, p, pts, cor.PREC_FP64)
+    err = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    print(name, "fp64 max rel err", err.max())
+    assert np.all(np.isfinite(got)) and err.max() < (1e-12 if ext == 0 else 1e-8)
+    got = cor.field(ms, p, pts, cor.PREC_MIXED)
+    r = np.linalg.norm(pts, axis=1) / RE
+    scale = np.linalg.norm(want, axis=1) + 30574.0 / r ** 3
+    err = np.linalg.norm(got - want, axis=1) / scale
+    print(name, "mixed max rel err", err.max(), "p99", np.quantile(err, 0.99))
+    assert np.all(np.isfinite(got)) and err.max() < 1e-3 and np.quantile(err, 0.99) < 1e-4

@@ -150,3 +150,47 @@ def test_cooperative_handover_many_scans(cor):
         d = np.abs(out[tl]["rc"] - out[-1]["rc"])
         print(tl, "max |dRc|", d.max(), "frac <= 0.1 GV", (d <= 0.1 + 1e-9).mean())
         assert (d <= 0.1 + 1e-9).mean() >= 0.8
+
+
+TS05_STORM = [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_ts05_site(cor, oracle, precision):
+    """IGRF + TS05 at the golden TS05 site (52N 30E, storm drivers, 1..20 GV in 0.25,
+    2e4 steps) against the oracle: FP64 at the FP64 bar; mixed with Rc within half the
+    penumbra width."""
+    osc = scan(52.0, 30.0, r0=1.00280, rig=1.0, dl=0.25, rk=20.0, nk1=100)
+    row, tg, cg, tc, cc = run_both(cor, oracle, osc, dict(internal=1, external=2), parmod=TS05_STORM,
+                                   step_limit=20000, precision=precision)
+    print("ts05", precision, cg, "cpu", cc)
+    if precision == 0:
+        compare(row, tg, cg, tc, cc, rc_tol=0.25)
+    else:
+        compare(row, tg, cg, tc, cc, rc_tol=max(0.25, 0.5 * (cc[1] - cc[0])), dir_tol=0.5, dir_margin=1.0,
+                exit_tol=5.0)
+
+
+def test_sh_ts05_site_fp64(cor, oracle):
+    """The 1-degree map's field (degree-20 SH core field + TS05 storm drivers, GSM frame
+    of 2020.0, no IGRF) at one site, FP64 against the oracle."""
+    rng = np.random.default_rng(11)
+    nm = 21
+    g = np.zeros((nm, nm))
+    h = np.zeros((nm, nm))
+    for n in range(1, nm):
+        sc = 30000.0 * 0.3 ** n
+        for m in range(n + 1):
+            g[n, m] = sc * rng.normal()
+            h[n, m] = sc * rng.normal() if m else 0.0
+    osc = scan(-20.0, 150.0, r0=1.00310, rig=1.0, dl=0.25, rk=20.0, nk1=100)
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=TS05_STORM)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    for k, v in enumerate(TS05_STORM):
+        po.PARMOD[k] = v
+    row = cor.scans_array([(osc.r0, osc.the0, osc.fi0, osc.the1, osc.fi1, osc.rig, osc.del_, osc.rk, osc.zn, osc.nk1)])
+    tg, cg = cor.trace(cor.model_set(cor.INTERNAL_NONE, cor.EXTERNAL_TS05, sh=(g, h)), p, row, step_limit=20000,
+                       precision=cor.PREC_FP64)
+    tc, cc = oracle.trace_scan(osc, L.models(internal=0, external=2, crustal=1, chaos=(g, h)), po, 20000)
+    print("sh+ts05", cg[0], "cpu", cc)
+    compare(row[0], tg, cg[0], tc, cc, rc_tol=0.25)
