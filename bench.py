@@ -12,8 +12,10 @@ Workload (one "step" = one pass over the whole batch, inputs resident in HBM):
   baseline runs with the same value).
 
 Printed: one JSON line (value = mixed-precision trajectories/s over the whole batch),
-with the FP64 GPU figures, the mixed-vs-FP64 cutoff agreement, the roofline of the
-tracer kernel and the reference CPU path timed on this host.
+with the FP64 GPU figures on every 16th site (the mixed path re-run on the same sites
+for a like-for-like time ratio; the cutoff agreement of the full-grid mixed run on
+those sites), the roofline of the tracer kernel and the reference CPU path timed on
+this host.
 """
 import argparse
 import json
@@ -73,7 +75,7 @@ def time_plan(plan, steps, warmup):
 def traffic_per_launch(n_traj):
     """HBM bytes per tracer launch from the committed FETCH_SIZE / WRITE_SIZE counter
     runs (rocprofv3 --pmc), each scaled by the same counter's reading for a plain
-    elementwise copy of known size; stored per trajectory and scaled to this launch."""
+    device-to-device copy of known size; stored per trajectory and scaled to this launch."""
     f = REPO / "profiles" / "traffic_trace_kernel_mixed_t96_grid5.json"
     if not f.exists():
         return None
@@ -82,53 +84,30 @@ def traffic_per_launch(n_traj):
 
 
 def copy_calibration(gib=1.0, reps=3):
-    """A plain elementwise copy (y = x + 0, float32) of known size, run a few times so
-    that a counter run sees its FETCH_SIZE / WRITE_SIZE next to the tracer's."""
-    import torch
-    n = int(gib * (1 << 30)) // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
+    """A plain device-to-device copy of known size (hipMemcpy, the HIP runtime's copy
+    kernel), run a few times so that a counter run sees its FETCH_SIZE / WRITE_SIZE
+    next to the tracer's."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")          # the runtime libcor_amd.so links
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    n = int(gib * (1 << 30))
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    ok = hip.hipMemset(src, 1, n) == 0
     for _ in range(reps):
-        torch.add(x, 0.0, out=y)
-    torch.cuda.synchronize()
-    print(f"[bench] copy calibration: {reps} x {n * 4} bytes read + {n * 4} bytes written",
+        ok = ok and hip.hipMemcpy(dst, src, n, 3) == 0       # hipMemcpyDeviceToDevice
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if not ok:
+        raise RuntimeError("copy calibration failed")
+    print(f"[bench] copy calibration: {reps} x {n} bytes copied device to device",
           file=sys.stderr, flush=True)
-    del x, y
-
-
-def cpu_baseline(sites, budget_s, nk1, step_limit):
-    """The reference itself (oracle/_ref/cor, built from its sources) on a bounded
-    sample of the same sites: whole sites, -p (OpenMP over the rigidity scan) on this
-    host's cores, until the time budget is spent.  Returns the rate and, per sampled
-    site, the reference's (Rl, Ru, Rc)."""
-    ref = REPO / "oracle" / "_ref" / "cor"
-    if not ref.exists():
-        return None, {}
-    rng = np.random.default_rng(12345)
-    order = rng.permutation(len(sites))
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    done, n_traj, t_total, cuts = 0, 0, 0.0, {}
-    with tempfile.TemporaryDirectory() as td:
-        for k in order:
-            r0, lat, lon = sites[k]
-            inp = Path(td) / "in.txt"
-            inp.write_text(f"0.01 -1 20.00\n{r0!r} {lat!r} {lon!r}\n{lat!r} {lon!r}\n"
-                           f"2020 1 1 1 0 0 0\n{nk1} 0 0 0.01\n-20 2 0 -2\n")
-            t0 = time.perf_counter()
-            res = subprocess.run([str(ref), "-p", "-i", "igrf14", "-e", "t96", "-s", str(step_limit), str(inp)],
-                                 check=True, capture_output=True, timeout=600, text=True)
-            t_total += time.perf_counter() - t0
-            lines = res.stdout.splitlines()
-            for j, ln in enumerate(lines):
-                if "CUTOFF with rigidities" in ln:
-                    cuts[int(k)] = tuple(float(v) for v in lines[j + 1].split())
-            done += 1
-            n_traj += 2000
-            if t_total >= budget_s and done >= 2:
-                break
-    return ({"value": n_traj / t_total, "unit": "trajectories/s", "cores": threads, "kind": "reference",
-             "sample": f"{done} seeded sites of the benchmark grid x 2000 rigidities "
-                       f"({n_traj} trajectories) in {t_total:.1f} s, oracle/_ref/cor -p"}, cuts)
 
 
 def main():
@@ -148,7 +127,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a 1 GiB elementwise copy (for counter calibration)")
+                    help="also run a 1 GiB device-to-device copy (for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the tracer shards by "
@@ -172,7 +151,10 @@ def main():
     _, cut_m = mixed.results(traj=False)
     mixed.close()
     print(f"[bench] mixed: {t_mixed*1e3:.1f} ms/step, kernel {k_mixed:.1f} ms (HIP events, mean of "
-          f"{args.steps}), {st_m.steps} steps, {st_m.attempts} attempts", file=sys.stderr, flush=True)
+          f"{args.steps}), {st_m.steps} steps, {st_m.attempts} attempts; lane fill "
+          f"{st_m.lane_iters / max(64 * st_m.wave_iters, 1):.3f}, queue drained at {st_m.drain_ms:.0f} ms, "
+          f"per-lane mode ended at {st_m.lane_end_ms:.0f} ms, {st_m.coop_steps} cooperative steps",
+          file=sys.stderr, flush=True)
 
     fp64 = None
     cut_d = None
@@ -186,17 +168,25 @@ def main():
         _, cut_d = p64.results(traj=False)
         p64.close()
         n64 = len(sub) * (n // len(scans))
+        # the mixed path on the same sites, for a like-for-like speed ratio (both runs
+        # of a site subset end with the same slowest trajectories)
+        msub = A.Plan(models, params, scans[sub], step_limit=args.step_limit, precision=A.PREC_MIXED,
+                      queue_order=args.queue_order, tail_lanes=args.tail_lanes)
+        t_msub, _ = time_plan(msub, 1, 0)
+        msub.close()
         drc = np.abs(cut_m["rc"][sub] - cut_d["rc"])
         fp64 = {"sites": int(len(sub)), "trajectories": int(n64), "value": n64 / t64,
                 "ms_per_step": t64 * 1e3, "kernel_ms": st_d.kernel_ms,
                 "steps": int(st_d.steps), "attempts": int(st_d.attempts),
-                "speedup_mixed_vs_fp64": (n / t_mixed) / (n64 / t64),
+                "mixed_ms_same_sites": t_msub * 1e3,
+                "speedup_mixed_vs_fp64_same_sites": t64 / t_msub,
                 "sites_within_0.01GV": float(np.mean(drc <= 0.01 + 1e-9)),
                 "max_abs_dRc_GV": float(drc.max()),
                 "mean_abs_dRl_GV": float(np.mean(np.abs(cut_m["rl"][sub] - cut_d["rl"]))),
                 "mean_abs_dRu_GV": float(np.mean(np.abs(cut_m["ru"][sub] - cut_d["ru"])))}
-        print(f"[bench] fp64: {t64*1e3:.1f} ms/step, kernel {st_d.kernel_ms:.1f} ms; "
-              f"sites within 0.01 GV: {fp64['sites_within_0.01GV']:.4f}", file=sys.stderr, flush=True)
+        print(f"[bench] fp64: {t64*1e3:.1f} ms/step, kernel {st_d.kernel_ms:.1f} ms (mixed on the same "
+              f"sites {t_msub*1e3:.1f} ms); sites within 0.01 GV: {fp64['sites_within_0.01GV']:.4f}",
+              file=sys.stderr, flush=True)
 
     roof = None
     fm = flop_model()
@@ -241,6 +231,12 @@ def main():
                    "queue_order": args.queue_order, "tail_lanes": args.tail_lanes},
         "kernel_ms": k_mixed,
         "integration_steps_per_s": st_m.steps / (k_mixed * 1e-3),
+        # where the tracer's time goes: mean live lanes per per-lane wavefront step,
+        # the time until the queue's last trajectory was started and until the last
+        # wavefront left the per-lane mode, and the cooperative-mode steps
+        "phases": {"lane_fill": st_m.lane_iters / max(64 * st_m.wave_iters, 1),
+                   "wave_steps": int(st_m.wave_iters), "drain_ms": st_m.drain_ms,
+                   "lane_end_ms": st_m.lane_end_ms, "coop_steps": int(st_m.coop_steps)},
         "fp64": fp64,
         "roofline": roof,
         "cpu_baseline": cpu,

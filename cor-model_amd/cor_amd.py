@@ -37,6 +37,7 @@ EXPORTED = [
     "cor_trace_plan_run", "cor_trace_plan_sync", "cor_trace_plan_count", "cor_trace_plan_results",
     "cor_trace_plan_stats", "cor_trace_plan_destroy", "cor_gpu_trace", "cor_gpu_field",
     "cor_scan_count", "cor_geodetic_to_geocentric", "cor_direction_to_geo", "cor_gpu_synchronize",
+    "cor_trace_plan_set_epoch", "cor_gpu_trace_series", "cor_refine_scan",
 ]
 
 
@@ -76,7 +77,8 @@ class Cutoff(C.Structure):
 class TraceStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("steps", C.c_uint64),
                 ("attempts", C.c_uint64), ("n_traj", C.c_int64), ("grid_blocks", C.c_int32),
-                ("block_threads", C.c_int32)]
+                ("block_threads", C.c_int32), ("wave_iters", C.c_uint64), ("lane_iters", C.c_uint64),
+                ("coop_steps", C.c_uint64), ("drain_ms", C.c_double), ("lane_end_ms", C.c_double)]
 
 
 TRAJ_DTYPE = np.dtype([("status", "<i4"), ("steps", "<u4"), ("attempts", "<u4"), ("pad", "<u4"),
@@ -124,9 +126,13 @@ def lib():
     L.cor_trace_plan_destroy.restype = None
     L.cor_gpu_trace.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p, C.c_int32,
                                 C.POINTER(TraceOpts), C.c_void_p, C.c_void_p]
+    L.cor_trace_plan_set_epoch.argtypes = [C.c_void_p, C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p]
+    L.cor_gpu_trace_series.argtypes = [C.POINTER(ModelSet), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                       C.POINTER(TraceOpts), C.c_int32, C.c_void_p, C.c_void_p]
     L.cor_gpu_field.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), DP, DP, C.c_int64, C.c_int32]
     L.cor_geodetic_to_geocentric.argtypes = [C.c_double, C.c_double, DP, DP]
     L.cor_direction_to_geo.argtypes = [C.c_double] * 4 + [DP, DP]
+    L.cor_refine_scan.argtypes = [C.POINTER(Scan), C.POINTER(Cutoff), C.c_int32, C.POINTER(Scan)]
     _lib = L
     return L
 
@@ -213,6 +219,11 @@ class Plan:
     def sync(self):
         check(lib().cor_trace_plan_sync(self.h), "plan_sync")
 
+    def set_epoch(self, params, stream=None):
+        """Re-point the plan at another epoch (same model kinds) before the next run."""
+        check(lib().cor_trace_plan_set_epoch(self.h, C.byref(self._models), C.byref(params),
+                                             C.c_void_p(stream) if stream else None), "plan_set_epoch")
+
     def results(self, traj=True):
         cut = np.zeros(len(self.scans), dtype=CUT_DTYPE)
         tr = np.zeros(self.n, dtype=TRAJ_DTYPE) if traj else None
@@ -246,6 +257,22 @@ def trace(models, params, scans, **kw):
         p.close()
 
 
+def trace_series(models, epochs, scans, in_flight=0, step_limit=1000000, tu_angle=0.01, mass=M_PROTON,
+                 precision=PREC_FP64, queue_order=0, tail_lanes=0, traj=True):
+    """The scans traced at every epoch (a list of ModelsParams): per-trajectory results
+    of shape (n_epochs, count) and cutoffs of shape (n_epochs, n_scans)."""
+    scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
+    count = sum(scan_count(r) for r in scans)
+    arr = (ModelsParams * len(epochs))(*epochs)
+    opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, tail_lanes, 0)
+    cut = np.zeros((len(epochs), len(scans)), dtype=CUT_DTYPE)
+    tr = np.zeros((len(epochs), count), dtype=TRAJ_DTYPE) if traj else None
+    check(lib().cor_gpu_trace_series(C.byref(models), C.cast(arr, C.c_void_p), len(epochs), scans.ctypes.data,
+                                     len(scans), C.byref(opts), in_flight, tr.ctypes.data if traj else None,
+                                     cut.ctypes.data), "cor_gpu_trace_series")
+    return tr, cut
+
+
 def field(models, params, gsm_m, precision=PREC_FP64):
     x = np.ascontiguousarray(gsm_m, dtype=np.float64).reshape(-1, 3)
     out = np.empty_like(x)
@@ -265,6 +292,19 @@ def direction_to_geo(lat_deg, lon_deg, zenith_deg, azimuth_deg):
     check(lib().cor_direction_to_geo(lat_deg, lon_deg, zenith_deg, azimuth_deg, C.byref(t), C.byref(f)),
           "direction")
     return t.value, f.value
+
+
+def refine_scans(rows, cuts, factor=10):
+    """cor_refine_scan per row: the scans re-tracing [Rl - del, Ru] at del / factor."""
+    out = np.zeros(len(rows), dtype=SCAN_DTYPE)
+    for i, (r, c) in enumerate(zip(rows, cuts)):
+        s = Scan(*[float(r[k]) for k in ("r0", "the0", "fi0", "the1", "fi1", "rig", "del_", "rk", "zn")],
+                 int(r["nk1"]), 0)
+        cc = Cutoff(float(c["rl"]), float(c["ru"]), float(c["rc"]), int(c["n_forbidden"]), int(c["n"]))
+        f = Scan()
+        check(lib().cor_refine_scan(C.byref(s), C.byref(cc), factor, C.byref(f)), "cor_refine_scan")
+        out[i] = (f.r0, f.the0, f.fi0, f.the1, f.fi1, f.rig, f.del_, f.rk, f.zn, f.nk1, 0)
+    return out
 
 
 def grid_sites(lat_step=5.0, lon_step=5.0, h_km=20.0):

@@ -22,6 +22,7 @@ thread_local char g_hip_error[256] = "";
 int g_device = -1;
 hipStream_t g_stream = nullptr;
 int g_num_cus = 0;
+double g_wall_khz = 0.;       // wall_clock64 rate of the device
 
 int fail_hip(hipError_t e, const char *what) {
     snprintf(g_hip_error, sizeof g_hip_error, "%s: %s", what, hipGetErrorString(e));
@@ -213,8 +214,14 @@ __global__ void __launch_bounds__(64) penumbra_kernel(const ScanDev *__restrict_
     }
 }
 
-__global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop) {
+__global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunStats *st) {
     *head = 0ull;
+    st->wave_iters = 0ull;
+    st->lane_iters = 0ull;
+    st->coop_steps = 0ull;
+    st->t_start = ~0ull;
+    st->t_drain = 0ull;
+    st->t_lane_end = 0ull;
     if (coop) {
         coop->res = 0ull;
         coop->head = 0ull;
@@ -253,8 +260,12 @@ struct cor_trace_plan {
     uint32_t run_id;
     CoopCtl *d_coop;
     ParkDev *d_park;
+    RunStats *d_run;
     hipEvent_t ev[3];
     bool ran;
+    EpochDev *h_epoch;           // pinned staging of set_epoch uploads
+    hipEvent_t ev_up;            // the last set_epoch upload has left h_epoch
+    bool up_pending;
 };
 
 extern "C" {
@@ -275,6 +286,9 @@ int cor_gpu_init(int device) {
         return COR_E_NO_DEVICE;
     }
     g_num_cus = prop.multiProcessorCount;
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    g_wall_khz = khz > 0 ? (double)khz : 0.;
     if (!g_stream) HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;
     return COR_OK;
@@ -323,8 +337,12 @@ void cor_trace_plan_destroy(cor_trace_plan *p) {
     (void)hipFree(p->d_stat);
     (void)hipFree(p->d_coop);
     (void)hipFree(p->d_park);
+    (void)hipFree(p->d_run);
     for (int k = 0; k < 3; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
+    if (p->up_pending) (void)hipEventSynchronize(p->ev_up);
+    if (p->ev_up) (void)hipEventDestroy(p->ev_up);
+    if (p->h_epoch) (void)hipHostFree(p->h_epoch);
     delete p;
 }
 
@@ -402,6 +420,7 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     PLAN_TRY(hipMemset(p->out.vc, 0, sizeof(double) * nt * 8));
     PLAN_TRY(hipMalloc(&p->d_cut, sizeof(cor_cutoff) * n_scans));
     PLAN_TRY(hipMalloc(&p->d_stat, sizeof(ScanStat) * n_scans));
+    PLAN_TRY(hipMalloc(&p->d_run, sizeof(RunStats)));
 
     if (opts->queue_order == 1 && total > 0) {
         // longest-expected first: trajectories near the lower end of each scan (low
@@ -476,7 +495,8 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.run_id = ++p->run_id;      // park records of earlier runs never match
     a.coop = p->d_coop;
     a.park = p->d_park;
-    reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop);
+    a.stats = p->d_run;
+    reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop, p->d_run);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(p->ev[0], s));
     if (p->n_items > 0) {
@@ -552,7 +572,84 @@ int cor_trace_plan_stats(cor_trace_plan *p, cor_trace_stats *st) {
     st->n_traj = p->n_items;
     st->grid_blocks = p->grid;
     st->block_threads = p->block;
+    RunStats r;
+    HIP_TRY(hipMemcpy(&r, p->d_run, sizeof r, hipMemcpyDeviceToHost));
+    st->wave_iters = r.wave_iters;
+    st->lane_iters = r.lane_iters;
+    st->coop_steps = r.coop_steps;
+    const double tick_ms = g_wall_khz > 0. ? 1. / g_wall_khz : 0.;
+    const bool started = r.t_start != ~0ull;
+    st->drain_ms = started && r.t_drain >= r.t_start ? (double)(r.t_drain - r.t_start) * tick_ms : 0.;
+    st->lane_end_ms = started && r.t_lane_end >= r.t_start ? (double)(r.t_lane_end - r.t_start) * tick_ms : 0.;
     return COR_OK;
+}
+
+int cor_trace_plan_set_epoch(cor_trace_plan *p, const cor_model_set *models, const ModelsParams *epoch,
+                             void *stream) {
+    if (!p || !epoch) return COR_E_INVALID;
+    int intk, ext, cru, rc;
+    if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
+    if (intk != p->intk || ext != p->ext || cru != p->cru) return COR_E_INVALID;
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    if (!p->h_epoch) {
+        HIP_TRY(hipHostMalloc((void **)&p->h_epoch, sizeof(EpochDev), hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&p->ev_up, hipEventDisableTiming));
+    }
+    // the staging block may still feed the previous upload
+    if (p->up_pending) HIP_TRY(hipEventSynchronize(p->ev_up));
+    p->up_pending = false;
+    build_epoch(models, epoch, *p->h_epoch);
+    HIP_TRY(hipMemcpyAsync(p->d_epoch, p->h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s));
+    if (p->coop_lanes > 0) HIP_TRY(build_coop_tables(p->d_epoch, s));
+    HIP_TRY(hipEventRecord(p->ev_up, s));
+    p->up_pending = true;
+    return COR_OK;
+}
+
+namespace {
+// streams of the epoch pipeline (slot 0 is the library stream); default priority
+constexpr int SERIES_MAX_IN_FLIGHT = 4;
+hipStream_t g_series_stream[SERIES_MAX_IN_FLIGHT] = {};
+}  // namespace
+
+int cor_gpu_trace_series(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
+                         const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
+                         int32_t in_flight, cor_traj *traj, cor_cutoff *cut) {
+    if (!epochs || n_epochs < 1 || !scans || n_scans < 1 || !opts || in_flight < 0) return COR_E_INVALID;
+    int rc = ensure_init();
+    if (rc) return rc;
+    int k_max = in_flight == 0 ? 3 : std::min(in_flight, SERIES_MAX_IN_FLIGHT);
+    k_max = std::min(k_max, n_epochs);
+    g_series_stream[0] = g_stream;
+    for (int k = 1; k < k_max; k++)
+        if (!g_series_stream[k]) HIP_TRY(hipStreamCreateWithFlags(&g_series_stream[k], hipStreamNonBlocking));
+    cor_trace_plan *plan[SERIES_MAX_IN_FLIGHT] = {};
+    int64_t count = 0;
+    // epoch e runs in slot e % k_max; before a slot takes its next epoch, the results
+    // of the previous one are copied out, so up to k_max epochs are on the device and
+    // the tail of one overlaps the bulk of the next
+    auto collect = [&](int slot, int e) -> int {
+        cor_traj *t = traj ? traj + (size_t)e * (size_t)count : nullptr;
+        cor_cutoff *c = cut ? cut + (size_t)e * (size_t)n_scans : nullptr;
+        return cor_trace_plan_results(plan[slot], t, c);
+    };
+    for (int e = 0; e < n_epochs && rc == COR_OK; e++) {
+        const int slot = e % k_max;
+        if (!plan[slot]) {
+            rc = cor_trace_plan_create(models, &epochs[e], scans, n_scans, opts, &plan[slot]);
+            if (rc == COR_OK) count = plan[slot]->n_items;
+        } else {
+            rc = collect(slot, e - k_max);
+            if (rc == COR_OK) rc = cor_trace_plan_set_epoch(plan[slot], models, &epochs[e], g_series_stream[slot]);
+        }
+        if (rc == COR_OK) rc = cor_trace_plan_run(plan[slot], g_series_stream[slot]);
+    }
+    for (int e = std::max(0, n_epochs - k_max); e < n_epochs && rc == COR_OK; e++) rc = collect(e % k_max, e);
+    for (int k = 0; k < k_max; k++) {
+        if (plan[k] && plan[k]->ran) (void)hipEventSynchronize(plan[k]->ev[2]);
+        cor_trace_plan_destroy(plan[k]);
+    }
+    return rc;
 }
 
 int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,

@@ -112,6 +112,17 @@ struct TraceOut {
     double *vc, *r, *td, *fei, *ast, *asf, *time, *length;
 };
 
+// Per-run counters of the tracer: how full the per-lane wavefronts were and when the
+// phases ended (times in wall_clock64 ticks; t_start starts at all ones).
+struct RunStats {
+    unsigned long long wave_iters;   // per-lane mode: loop iterations summed over wavefronts
+    unsigned long long lane_iters;   // per-lane mode: live lanes summed over those iterations
+    unsigned long long coop_steps;   // cooperative mode: field evaluations (one per wavefront step)
+    unsigned long long t_start;      // earliest wavefront start
+    unsigned long long t_drain;      // the queue's last slot taken
+    unsigned long long t_lane_end;   // last wavefront to leave the per-lane mode
+};
+
 struct TraceArgs {
     const EpochDev *epoch;
     const ScanDev *scans;
@@ -133,6 +144,7 @@ struct TraceArgs {
     uint32_t run_id;
     CoopCtl *coop;
     ParkDev *park;
+    RunStats *stats;
 };
 
 }  // namespace cor

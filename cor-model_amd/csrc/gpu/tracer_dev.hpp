@@ -297,6 +297,26 @@ __device__ __forceinline__ int find_scan(const ScanDev *__restrict__ scans, int 
 
 // src/trajectory.c:81,91-112: exit point (car_to_sph) and asymptotic direction of an
 // allowed trajectory, written to its result record.
+// A trajectory's end: status and counters; the exit record of a forbidden one is
+// zeroed, so every run of a plan writes every output (a re-run at another epoch leaves
+// nothing of the previous one).
+__device__ __forceinline__ void write_end(const TraceArgs &a, int64_t gid, int status, unsigned int step,
+                                          uint32_t attempts) {
+    a.out.status[gid] = status;
+    a.out.steps[gid] = step;
+    a.out.attempts[gid] = attempts;
+    if (status != ST_ALLOWED) {
+        a.out.vc[gid] = 0.;
+        a.out.r[gid] = 0.;
+        a.out.td[gid] = 0.;
+        a.out.fei[gid] = 0.;
+        a.out.ast[gid] = 0.;
+        a.out.asf[gid] = 0.;
+        a.out.time[gid] = 0.;
+        a.out.length[gid] = 0.;
+    }
+}
+
 __device__ __forceinline__ void write_exit(const TraceArgs &a, int64_t gid, v3<double> pos, v3<double> vel,
                                            double v, double r, double time, double alength) {
     const double rho2 = pos.x * pos.x + pos.y * pos.y;
@@ -550,7 +570,9 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
         uint32_t attempts = P->attempts;
         const int nk1 = P->nk1;
         int nk0 = nk1;
+        unsigned long long evals = 0;
         for (;;) {
+            evals++;
             const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
             const v3<float> f = coop_field<INT, EXT>(E.G, E.H, E.REC, E.Af, E.t96, T, pos, gsm, lane);
             const double fx = f.x, fy = f.y, fz = f.z;
@@ -559,9 +581,8 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
                                               gid, lane == 0);
             if (status >= 0) {
                 if (lane == 0) {
-                    a.out.status[gid] = status;
-                    a.out.steps[gid] = step;
-                    a.out.attempts[gid] = attempts;
+                    write_end(a, gid, status, step, attempts);
+                    atomicAdd(&a.stats->coop_steps, evals);
                 }
                 break;
             }
@@ -611,6 +632,9 @@ trace_kernel(TraceArgs a) {
     // cooperative workers waiting for work (tickets taken minus slots reserved), as
     // read one step earlier
     long long idle_seen = 0;
+    // fill of this wavefront's per-lane loop
+    unsigned long long wave_iters = 0, lane_iters = 0;
+    if (lane == 0) atomicMin(&a.stats->t_start, (unsigned long long)wall_clock64());
 
     for (;;) {
         // ---- 1. refill ended lanes from the queue
@@ -621,7 +645,12 @@ trace_kernel(TraceArgs a) {
                 unsigned long long base = 0;
                 if (lane == leader) base = atomicAdd(a.head, (unsigned long long)__popcll(idle));
                 base = __shfl(base, leader);
-                if (base + __popcll(idle) >= (unsigned long long)a.n_items) queue_open = false;
+                if (base + __popcll(idle) >= (unsigned long long)a.n_items) {
+                    queue_open = false;
+                    // this wavefront took the queue's last slot
+                    if (lane == leader && base < (unsigned long long)a.n_items)
+                        a.stats->t_drain = (unsigned long long)wall_clock64();
+                }
                 if (!active) {
                     const unsigned long long slot = base + __popcll(idle & ((1ull << lane) - 1ull));
                     if (slot < (unsigned long long)a.n_items) {
@@ -669,7 +698,10 @@ trace_kernel(TraceArgs a) {
                 }
             }
         }
-        if (!__any(active)) break;
+        const uint64_t live_mask = __ballot(active);
+        if (!live_mask) break;
+        wave_iters++;
+        lane_iters += __popcll(live_mask);
 
         if (active) {
             // ---- 2. field at the current point (src/trajectory.c:37-38,147-148); the
@@ -695,12 +727,15 @@ trace_kernel(TraceArgs a) {
                 status = attempts_mixed(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
             }
             if (status >= 0) {
-                a.out.status[gid] = status;
-                a.out.steps[gid] = step;
-                a.out.attempts[gid] = attempts;
+                write_end(a, gid, status, step, attempts);
                 active = false;
             }
         }
+    }
+    if (lane == 0) {
+        atomicAdd(&a.stats->wave_iters, wave_iters);
+        atomicAdd(&a.stats->lane_iters, lane_iters);
+        atomicMax(&a.stats->t_lane_end, (unsigned long long)wall_clock64());
     }
     if constexpr (kCoop) {
         if (a.coop_lanes > 0) {

@@ -48,3 +48,22 @@ int cor_direction_to_geo(double lat_deg, double lon_deg, double zenith_deg, doub
     *fi1_deg = f;
     return COR_OK;
 }
+
+/* Penumbra refinement: the scan that re-traces [Rl - del, Ru] of a finished scan at
+ * del / factor, clamped to the coarse scan's range (a reference run of the same
+ * station with that rig, rk and del traces the same rigidities). */
+int cor_refine_scan(const cor_scan *coarse, const cor_cutoff *cut, int32_t factor, cor_scan *fine) {
+    if (!coarse || !cut || !fine || factor < 1 || !(coarse->del > 0.) || !isfinite(cut->rl) ||
+        !isfinite(cut->ru))
+        return COR_E_INVALID;
+    double lo = cut->rl - coarse->del;
+    if (lo < coarse->rig) lo = coarse->rig;
+    double hi = cut->ru;
+    if (hi > coarse->rk) hi = coarse->rk;
+    if (hi < lo) hi = lo;
+    *fine = *coarse;
+    fine->rig = lo;
+    fine->rk = hi;
+    fine->del = coarse->del / factor;
+    return COR_OK;
+}

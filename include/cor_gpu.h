@@ -108,6 +108,11 @@ typedef struct {
     int64_t n_traj;
     int32_t grid_blocks;
     int32_t block_threads;
+    uint64_t wave_iters;      /* per-lane mode: wavefront loop iterations (steps), summed */
+    uint64_t lane_iters;      /* per-lane mode: live lanes summed over those iterations */
+    uint64_t coop_steps;      /* cooperative mode: wavefront steps (one field each) */
+    double drain_ms;          /* first wavefront start -> the queue's last slot taken */
+    double lane_end_ms;       /* first wavefront start -> last wavefront leaves per-lane mode */
 } cor_trace_stats;
 
 typedef struct cor_trace_plan cor_trace_plan;
@@ -134,10 +139,27 @@ int cor_trace_plan_results(cor_trace_plan *plan, cor_traj *traj, cor_cutoff *cut
 int cor_trace_plan_stats(cor_trace_plan *plan, cor_trace_stats *stats);
 void cor_trace_plan_destroy(cor_trace_plan *plan);
 
+/* Replace the plan's epoch (what igrf14_recalc and the model input lines fill for
+ * one run of the reference, src/main.c:38-75) with the same model kinds, so the scans
+ * can be re-traced at another time or for other drivers without rebuilding the plan.
+ * The upload is ordered on `stream` (NULL = the library's stream): run the plan on
+ * the same stream.  Results of the previous run must be copied out first. */
+int cor_trace_plan_set_epoch(cor_trace_plan *plan, const cor_model_set *models,
+                             const ModelsParams *epoch, void *stream);
+
 /* create + run + results + destroy */
 int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
                   int32_t n_scans, const cor_trace_opts *opts, cor_traj *traj,
                   cor_cutoff *cut);
+
+/* The same scans traced at each of n_epochs epochs (the reference runs its whole
+ * cutoff path once per epoch: src/main.c:38-86 per input file).  Up to in_flight
+ * epochs (0 = 3, at most 4) run at once on separate streams, so the slow tail of one
+ * epoch overlaps the next.  traj (nullable) receives n_epochs x count entries, epoch
+ * major; cut receives n_epochs x n_scans. */
+int cor_gpu_trace_series(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
+                         const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
+                         int32_t in_flight, cor_traj *traj, cor_cutoff *cut);
 
 /* calculate_field over n points: GSM position [m] (n x 3) -> GEO field [nT] (n x 3). */
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
@@ -154,6 +176,11 @@ int64_t cor_scan_count(const cor_scan *scan);
 int cor_geodetic_to_geocentric(double lat_deg, double h_km, double *r0_re, double *lat_c_deg);
 int cor_direction_to_geo(double lat_deg, double lon_deg, double zenith_deg, double azimuth_deg,
                          double *the1_deg, double *fi1_deg);
+
+/* Penumbra refinement (no counterpart in the reference, which is re-run with a finer
+ * del by hand): the scan covering [Rl - del, Ru] of a traced scan at del / factor,
+ * clamped to the coarse scan's [rig, rk]. */
+int cor_refine_scan(const cor_scan *coarse, const cor_cutoff *cut, int32_t factor, cor_scan *fine);
 
 #ifdef __cplusplus
 }

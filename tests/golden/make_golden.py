@@ -7,6 +7,7 @@ reference sources (make -C oracle ref first):
 
     python tests/golden/make_golden.py          (everything)
     python tests/golden/make_golden.py ts05     (the TS05 fixtures only)
+    python tests/golden/make_golden.py chaos    (the SH synthesis fixtures only)
 """
 import sys
 from pathlib import Path
@@ -76,9 +77,41 @@ def ts05_fixtures():
     print(name, txt.strip().splitlines()[-1])
 
 
+def seeded_sh(seed, nm=21):
+    """A degree nm-1 table with an IGRF-like spectrum (row n, column m, nT)."""
+    rng = np.random.default_rng(seed)
+    g = np.zeros((nm, nm))
+    h = np.zeros((nm, nm))
+    for n in range(1, nm):
+        sc = 30000.0 * 0.3 ** n
+        for m in range(n + 1):
+            g[n, m] = sc * rng.normal()
+            h[n, m] = sc * rng.normal() if m else 0.0
+    return g, h
+
+
+def chaos_fixtures():
+    """chaos_geo (lib/chaos/chaos.c:120-227) with two seeded tables (degree 20 and 23,
+    the largest the device takes) at SH test points: the reference's own outputs."""
+    ref = L.RefLib()
+    rng = np.random.default_rng(20240507)
+    pts = sh_points(rng, 2000)
+    out = {"chaos_pts": pts}
+    for k, (seed, nm) in enumerate([(11, 21), (12, 24)]):
+        g, h = seeded_sh(seed, nm)
+        out[f"chaos_g{k}"] = g
+        out[f"chaos_h{k}"] = h
+        out[f"chaos_b{k}"] = ref.chaos(g, h, pts)
+    np.savez_compressed(HERE / "chaos_vectors.npz", **out)
+    print("chaos vectors", {k: v.shape for k, v in out.items()})
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "ts05":
         ts05_fixtures()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "chaos":
+        chaos_fixtures()
         return
     ref = L.RefLib()
     rng = np.random.default_rng(20240501)
@@ -120,6 +153,7 @@ def main():
         (HERE / f"{name}.out").write_text(txt)
         print(name, txt.strip().splitlines()[-1])
     ts05_fixtures()
+    chaos_fixtures()
 
 
 if __name__ == "__main__":

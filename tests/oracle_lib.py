@@ -229,6 +229,23 @@ class RefLib:
         pm[:len(parmod)] = parmod
         return self._pointwise(self.lib.ts05_gsm, xyz, _dp(pm))
 
+    def chaos(self, g, h, xyz):
+        """chaos_geo (lib/chaos/chaos.c:120-227) with a caller's [nm][nm] G/H table."""
+
+        class Coefs(C.Structure):
+            _fields_ = [("G", C.c_void_p), ("H", C.c_void_p), ("nm_max", C.c_uint32)]
+
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        co = Coefs(g.ctypes.data, h.ctypes.data, g.shape[0])
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.empty_like(xyz)
+        bx, by, bz = C.c_double(), C.c_double(), C.c_double()
+        for i, (x, y, z) in enumerate(xyz):
+            self.lib.chaos_geo(x, y, z, C.addressof(co), C.byref(bx), C.byref(by), C.byref(bz))
+            out[i] = (bx.value, by.value, bz.value)
+        return out
+
 
 def have_ref():
     return REF_SO.exists() and REF_BIN.exists()

@@ -147,3 +147,41 @@ def test_gpu_calls_fail_cleanly_without_a_gpu():
     rc = lib.cor_gpu_field(C.byref(A.model_set()), C.byref(p), np.zeros(3).ctypes.data_as(A.DP),
                            out.ctypes.data_as(A.DP), 1, 0)
     assert rc == A.COR_E_NO_DEVICE
+
+
+def test_epoch_series_argument_checks():
+    """Invalid arguments of the epoch-series entry points are rejected before any
+    device work (the same checks run on a GPU host)."""
+    lib = A.lib()
+    p = A.recalc(2020)
+    ms = A.model_set()
+    assert lib.cor_trace_plan_set_epoch(None, C.byref(ms), C.byref(p), None) == A.COR_E_INVALID
+    scans = A.scans_array([(1.0, 45.0, 0.0, 45.0, 0.0, 1.0, 1.0, 3.0, -1.0, 100)])
+    opts = A.TraceOpts(1000, 0.01, A.M_PROTON, A.PREC_FP64, 0, 0, 0)
+    arr = (A.ModelsParams * 1)(p)
+    # no epochs, no scans, negative in-flight count
+    assert lib.cor_gpu_trace_series(C.byref(ms), C.cast(arr, C.c_void_p), 0, scans.ctypes.data, 1,
+                                    C.byref(opts), 0, None, None) == A.COR_E_INVALID
+    assert lib.cor_gpu_trace_series(C.byref(ms), C.cast(arr, C.c_void_p), 1, scans.ctypes.data, 0,
+                                    C.byref(opts), 0, None, None) == A.COR_E_INVALID
+    assert lib.cor_gpu_trace_series(C.byref(ms), C.cast(arr, C.c_void_p), 1, scans.ctypes.data, 1,
+                                    C.byref(opts), -1, None, None) == A.COR_E_INVALID
+
+
+def test_refine_scan():
+    """[Rl - del, Ru] at del / factor, clamped to the coarse scan's range."""
+    row = A.scans_array([(1.0, 45.0, 0.0, 45.0, 0.0, 0.01, 0.01, 20.0, -1.0, 100)])
+    cut = np.zeros(1, dtype=A.CUT_DTYPE)
+    cut[0] = (3.72, 4.77, 4.2, 0, 2000)
+    f = A.refine_scans(row, cut, 10)[0]
+    assert np.isclose(f["rig"], 3.71) and f["rk"] == 4.77 and np.isclose(f["del_"], 0.001)
+    assert A.scan_count(f) == 1061
+    assert (f["r0"], f["the0"], f["fi0"], f["zn"], f["nk1"]) == (1.0, 45.0, 0.0, -1.0, 100)
+    cut[0] = (0.01, 0.01, 0.01, 0, 2000)                     # everything allowed
+    f = A.refine_scans(row, cut, 10)[0]
+    assert f["rig"] == 0.01 and f["rk"] == 0.01 and A.scan_count(f) == 1
+    cut[0] = (20.0, 20.01, 20.0, 2000, 2000)                 # everything forbidden
+    f = A.refine_scans(row, cut, 10)[0]
+    assert np.isclose(f["rig"], 19.99) and f["rk"] == 20.0 and A.scan_count(f) == 11
+    with pytest.raises(A.CorError):
+        A.refine_scans(row, cut, 0)

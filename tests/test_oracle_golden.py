@@ -2,7 +2,8 @@
 tests/golden/make_golden.py from the reference compiled from its sources).
 
 Same compiler flags and the same libm: the restatement is expected to be bit-identical
-to the reference -- IGRF-14 epoch setup, IGRF synthesis, T96, TS05, and the complete
+to the reference -- IGRF-14 epoch setup, IGRF synthesis, T96, TS05, the generic SH
+synthesis (CHAOS form) with seeded tables, and the complete
 text of cutoff runs (every allowed-trajectory line and the Rl/Ru/Rc summary).
 """
 from pathlib import Path
@@ -45,6 +46,18 @@ def test_ts05_bitwise(oracle, k):
     tv = np.load(GOLD / "ts05_vectors.npz")
     got = oracle.ts05(tv[f"ts05_params{k}"], tv["ts05_pts"])
     want = tv[f"ts05_b{k}"]
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), np.argwhere(~same)[:5]
+
+
+@pytest.mark.parametrize("k", range(2))
+def test_chaos_bitwise(oracle, k):
+    """chaos_geo (lib/chaos/chaos.c:120-227) with seeded degree-20 and degree-23 tables,
+    bit for bit, including the NaN the reference returns exactly on the polar axis."""
+    cv = np.load(GOLD / "chaos_vectors.npz")
+    got = oracle.chaos(cv[f"chaos_g{k}"], cv[f"chaos_h{k}"], cv["chaos_pts"])
+    want = cv[f"chaos_b{k}"]
+    assert np.isnan(want).any()
     same = (got == want) | (np.isnan(got) & np.isnan(want))
     assert same.all(), np.argwhere(~same)[:5]
 
