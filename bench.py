@@ -235,7 +235,10 @@ def main():
         # the time until the queue's last trajectory was started and until the last
         # wavefront left the per-lane mode, and the cooperative-mode steps
         "phases": {"lane_fill": st_m.lane_iters / max(64 * st_m.wave_iters, 1),
-                   "wave_steps": int(st_m.wave_iters), "drain_ms": st_m.drain_ms,
+                   "wave_steps": int(st_m.wave_iters),
+                   "lane_fill_queue_open": st_m.lane_iters_open / max(64 * st_m.wave_iters_open, 1),
+                   "wave_steps_queue_open": int(st_m.wave_iters_open),
+                   "lane_steps_queue_open": int(st_m.lane_iters_open), "drain_ms": st_m.drain_ms,
                    "lane_end_ms": st_m.lane_end_ms, "coop_steps": int(st_m.coop_steps)},
         "fp64": fp64,
         "roofline": roof,

@@ -78,7 +78,8 @@ class TraceStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("steps", C.c_uint64),
                 ("attempts", C.c_uint64), ("n_traj", C.c_int64), ("grid_blocks", C.c_int32),
                 ("block_threads", C.c_int32), ("wave_iters", C.c_uint64), ("lane_iters", C.c_uint64),
-                ("coop_steps", C.c_uint64), ("drain_ms", C.c_double), ("lane_end_ms", C.c_double)]
+                ("coop_steps", C.c_uint64), ("wave_iters_open", C.c_uint64), ("lane_iters_open", C.c_uint64),
+                ("drain_ms", C.c_double), ("lane_end_ms", C.c_double)]
 
 
 TRAJ_DTYPE = np.dtype([("status", "<i4"), ("steps", "<u4"), ("attempts", "<u4"), ("pad", "<u4"),

@@ -219,6 +219,8 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
     st->wave_iters = 0ull;
     st->lane_iters = 0ull;
     st->coop_steps = 0ull;
+    st->wave_iters_open = 0ull;
+    st->lane_iters_open = 0ull;
     st->t_start = ~0ull;
     st->t_drain = 0ull;
     st->t_lane_end = 0ull;
@@ -232,6 +234,9 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
 // once the queue is drained, wavefronts with at most this many live trajectories hand
 // them to the cooperative mode (fuller ones do so when enough workers are idle)
 constexpr int COOP_LANES_DEFAULT = 4;
+
+// the mixed T96 evaluator reads its interconnection series from the lane tables
+bool needs_coop_tables(int precision, int ext) { return precision != COR_PREC_FP64 && ext == EXT_T96; }
 
 hipError_t build_coop_tables(EpochDev *d_epoch, hipStream_t s) {
     void *args[] = {&d_epoch};
@@ -456,11 +461,16 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
         grid = std::min<int64_t>(p->coop_lanes > 0 ? total : need, (int64_t)per_cu * 4 * g_num_cus);
     }
     p->grid = (int)std::max<int64_t>(grid, 1);
-    if (p->coop_lanes > 0) {
-        // cooperative tail: lane tables of the epoch, the handover counters and one park
-        // slot per lane of the grid (a lane parks at most one trajectory per run)
-        const size_t slots = (size_t)p->grid * p->block;
+    if (p->coop_lanes > 0 || needs_coop_tables(p->prec, ext)) {
+        // lane tables of the epoch: the cooperative evaluator's, whose T96
+        // interconnection series the per-lane mixed path also reads
         PLAN_TRY(build_coop_tables(p->d_epoch, g_stream));
+        PLAN_TRY(hipStreamSynchronize(g_stream));
+    }
+    if (p->coop_lanes > 0) {
+        // cooperative tail: the handover counters and one park slot per lane of the
+        // grid (a lane parks at most one trajectory per run)
+        const size_t slots = (size_t)p->grid * p->block;
         PLAN_TRY(hipMalloc(&p->d_coop, sizeof(CoopCtl)));
         PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * slots));
         PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * slots, g_stream));
@@ -577,6 +587,8 @@ int cor_trace_plan_stats(cor_trace_plan *p, cor_trace_stats *st) {
     st->wave_iters = r.wave_iters;
     st->lane_iters = r.lane_iters;
     st->coop_steps = r.coop_steps;
+    st->wave_iters_open = r.wave_iters_open;
+    st->lane_iters_open = r.lane_iters_open;
     const double tick_ms = g_wall_khz > 0. ? 1. / g_wall_khz : 0.;
     const bool started = r.t_start != ~0ull;
     st->drain_ms = started && r.t_drain >= r.t_start ? (double)(r.t_drain - r.t_start) * tick_ms : 0.;
@@ -600,7 +612,7 @@ int cor_trace_plan_set_epoch(cor_trace_plan *p, const cor_model_set *models, con
     p->up_pending = false;
     build_epoch(models, epoch, *p->h_epoch);
     HIP_TRY(hipMemcpyAsync(p->d_epoch, p->h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s));
-    if (p->coop_lanes > 0) HIP_TRY(build_coop_tables(p->d_epoch, s));
+    if (p->coop_lanes > 0 || needs_coop_tables(p->prec, p->ext)) HIP_TRY(build_coop_tables(p->d_epoch, s));
     HIP_TRY(hipEventRecord(p->ev_up, s));
     p->up_pending = true;
     return COR_OK;
@@ -686,7 +698,8 @@ int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const 
     if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 3 * n);
     if (e == hipSuccess) e = hipMemcpyAsync(d_e, &E, sizeof(EpochDev), hipMemcpyHostToDevice, g_stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_in, gsm_m, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g_stream);
-    if (e == hipSuccess && precision == COR_PREC_MIXED_WAVE) e = build_coop_tables(d_e, g_stream);
+    if (e == hipSuccess && (precision == COR_PREC_MIXED_WAVE || needs_coop_tables(precision, ext)))
+        e = build_coop_tables(d_e, g_stream);
     if (e == hipSuccess) {
         FieldBatchArgs a;
         a.epoch = d_e;

@@ -118,6 +118,8 @@ struct RunStats {
     unsigned long long wave_iters;   // per-lane mode: loop iterations summed over wavefronts
     unsigned long long lane_iters;   // per-lane mode: live lanes summed over those iterations
     unsigned long long coop_steps;   // cooperative mode: field evaluations (one per wavefront step)
+    unsigned long long wave_iters_open;   // the part of wave_iters while the queue was open
+    unsigned long long lane_iters_open;   // the part of lane_iters while the queue was open
     unsigned long long t_start;      // earliest wavefront start
     unsigned long long t_drain;      // the queue's last slot taken
     unsigned long long t_lane_end;   // last wavefront to leave the per-lane mode

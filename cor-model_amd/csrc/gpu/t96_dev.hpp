@@ -239,6 +239,37 @@ __device__ __forceinline__ v3<T> intercon(T x, T y, T z) {
     return b;
 }
 
+// Mixed path: the interconnection field of lib/t96/t96.c:236-271 from the series of the
+// cooperative evaluator's table (three y harmonics, 24 floats each, built on the device
+// from the epoch with rimfampl folded in; the k = 1, 2 pair of each harmonic, whose
+// terms cancel by four orders of magnitude, as sums of terms that do not).  The z
+// harmonics r_0 and r_2 are shared by the three rows.  Returns rimfampl times the
+// unrotated field.
+__device__ __forceinline__ v3<float> intercon_series(const float *__restrict__ tab, float X, float Y, float Z) {
+    float sz, cz, s2, c2;
+    Dm<float>::sincos(Z * tab[1], &sz, &cz);
+    Dm<float>::sincos(Z * tab[6], &s2, &c2);
+    const float z2 = Z * Z;
+    v3<float> b = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float *c = tab + 24 * i;
+        float sy, cy;
+        Dm<float>::sincos(Y * c[0], &sy, &cy);
+        const float e0 = c[3] * Dm<float>::exp(X * c[2]);
+        const float e1 = Dm<float>::exp(X * c[4]);
+        const float u = X * c[5];
+        const float del = u * (1.f + u * (0.5f + u * (1.f / 6.f)));
+        const float sx = Z * (c[10] - z2 * (c[11] - z2 * (c[12] - z2 * c[13]))) + c[7] * del * s2;
+        const float sy2 = Z * (c[14] - z2 * (c[15] - z2 * (c[16] - z2 * c[17]))) + c[8] * del * s2;
+        const float sz2 = c[14] - z2 * (c[18] - z2 * (c[19] - z2 * c[20])) + c[9] * del * c2;
+        b.x += -cy * (c[2] * e0 * sz + e1 * sx);
+        b.y += c[0] * sy * (e0 * sz + e1 * sy2);
+        b.z += -cy * (c[1] * e0 * cz + e1 * sz2);
+    }
+    return b;
+}
+
 // Conical current-sheet kernel shared by ringcurr_96 (lib/t96/t96.c:342-375) and
 // taildisk_t96 (:400-436): s1, s2 distances to the ring, and the derivatives of the
 // vector-potential factor AS with respect to s1, s2.
@@ -1016,8 +1047,9 @@ __device__ __forceinline__ v3<T> r2birk(T x, T y, T z) {
 }  // namespace t96
 
 // lib/t96/t96.c:4-111: the whole model at zero tilt.
+// icon: the mixed path's interconnection table (intercon_series); unused in FP64.
 template <class T>
-__device__ __forceinline__ v3<T> t96_field(const T96Params<T> &P, T x, T y, T z) {
+__device__ __forceinline__ v3<T> t96_field(const T96Params<T> &P, T x, T y, T z, const float *icon = nullptr) {
     using namespace t96;
     const T ys = y * P.ct - z * P.st, zs = z * P.ct + y * P.st;
     const T ysd = ys / T(10.0);
@@ -1097,21 +1129,23 @@ __device__ __forceinline__ v3<T> t96_field(const T96Params<T> &P, T x, T y, T z)
     COR_SEQ();
     {
         // The interconnection field is a fit whose terms cancel by four orders of
-        // magnitude (coefficients ~1e7 on sin(z r_k) with r_k ~ 1e-3): evaluated in
-        // FP64 in both paths.
-        v3<T> im;
-        if (Dm<T>::fp64) {
-            im = intercon(xx, ys * P.xappa, zs * P.xappa);
+        // magnitude (coefficients ~1e7 on sin(z r_k) with r_k ~ 1e-3): the reference's
+        // sum in FP64, the cancellation-free series in FP32.
+        if constexpr (Dm<T>::fp64) {
+            v3<T> im = intercon(xx, ys * P.xappa, zs * P.xappa);
+            pin(im);
+            const T rimfy = im.y * P.ct + im.z * P.st;
+            const T rimfz = im.z * P.ct - im.y * P.st;
+            fx = fx + P.rimfampl * im.x;
+            fy = fy + P.rimfampl * rimfy;
+            fz = fz + P.rimfampl * rimfz;
         } else {
-            const v3<double> imd = intercon<double>((double)xx, (double)(ys * P.xappa), (double)(zs * P.xappa));
-            im = mk3<T>((T)imd.x, (T)imd.y, (T)imd.z);
+            v3<float> im = intercon_series(icon, xx, ys * P.xappa, zs * P.xappa);
+            pin(im);
+            fx = fx + im.x;
+            fy = fy + (im.y * P.ct + im.z * P.st);
+            fz = fz + (im.z * P.ct - im.y * P.st);
         }
-        pin(im);
-        const T rimfy = im.y * P.ct + im.z * P.st;
-        const T rimfz = im.z * P.ct - im.y * P.st;
-        fx = fx + P.rimfampl * im.x;
-        fy = fy + P.rimfampl * rimfy;
-        fz = fz + P.rimfampl * rimfz;
     }
     if (sigma < T(1.08 - 0.005)) return mk3<T>(fx, fy, fz);
     const T fint = T(0.5) * (T(1.0) - (sigma - T(1.08)) / T(0.005));

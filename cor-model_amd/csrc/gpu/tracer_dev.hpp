@@ -42,6 +42,7 @@ struct EpochView {
     const TS05Params<double> *ts05d;     // FP64 constants of the shield sums
     const FT *shG, *shH;                 // SH model tables in HBM (uniform indices)
     int sh_nm;
+    const float *icon;                   // T96 interconnection series table (mixed path)
 };
 __device__ __forceinline__ const double *sh_tab(const EpochDev *E, int k, double) { return k ? E->shH : E->shG; }
 __device__ __forceinline__ const float *sh_tab(const EpochDev *E, int k, float) { return k ? E->shHf : E->shGf; }
@@ -83,6 +84,7 @@ __device__ __forceinline__ EpochView<FT> stage_epoch(const EpochDev *__restrict_
     V.shG = sh_tab(E, 0, FT(0));
     V.shH = sh_tab(E, 1, FT(0));
     V.sh_nm = E->sh_nm;
+    V.icon = &E->coop.icon[0][0];
     return V;
 }
 
@@ -152,7 +154,7 @@ __device__ __forceinline__ v3<float> field_mixed(const EpochView<float> &E, v3<d
     }
     if (EXT == EXT_T96) {
         const float x = (float)(gsm_m.x / K_RE), y = (float)(gsm_m.y / K_RE), z = (float)(gsm_m.z / K_RE);
-        const v3<float> bg = rot_gsm_to_geo_f(E.Af, t96_field<float>(E.t96, x, y, z));
+        const v3<float> bg = rot_gsm_to_geo_f(E.Af, t96_field<float>(E.t96, x, y, z, E.icon));
         tot.x += bg.x;
         tot.y += bg.y;
         tot.z += bg.z;
@@ -633,7 +635,7 @@ trace_kernel(TraceArgs a) {
     // read one step earlier
     long long idle_seen = 0;
     // fill of this wavefront's per-lane loop
-    unsigned long long wave_iters = 0, lane_iters = 0;
+    unsigned long long wave_iters = 0, lane_iters = 0, wave_iters_open = 0, lane_iters_open = 0;
     if (lane == 0) atomicMin(&a.stats->t_start, (unsigned long long)wall_clock64());
 
     for (;;) {
@@ -702,6 +704,10 @@ trace_kernel(TraceArgs a) {
         if (!live_mask) break;
         wave_iters++;
         lane_iters += __popcll(live_mask);
+        if (queue_open) {
+            wave_iters_open++;
+            lane_iters_open += __popcll(live_mask);
+        }
 
         if (active) {
             // ---- 2. field at the current point (src/trajectory.c:37-38,147-148); the
@@ -735,6 +741,8 @@ trace_kernel(TraceArgs a) {
     if (lane == 0) {
         atomicAdd(&a.stats->wave_iters, wave_iters);
         atomicAdd(&a.stats->lane_iters, lane_iters);
+        atomicAdd(&a.stats->wave_iters_open, wave_iters_open);
+        atomicAdd(&a.stats->lane_iters_open, lane_iters_open);
         atomicMax(&a.stats->t_lane_end, (unsigned long long)wall_clock64());
     }
     if constexpr (kCoop) {

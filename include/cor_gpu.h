@@ -111,6 +111,8 @@ typedef struct {
     uint64_t wave_iters;      /* per-lane mode: wavefront loop iterations (steps), summed */
     uint64_t lane_iters;      /* per-lane mode: live lanes summed over those iterations */
     uint64_t coop_steps;      /* cooperative mode: wavefront steps (one field each) */
+    uint64_t wave_iters_open; /* the part of wave_iters while the queue still had work */
+    uint64_t lane_iters_open; /* the part of lane_iters while the queue still had work */
     double drain_ms;          /* first wavefront start -> the queue's last slot taken */
     double lane_end_ms;       /* first wavefront start -> last wavefront leaves per-lane mode */
 } cor_trace_stats;
