@@ -128,6 +128,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (for counter calibration)")
+    ap.add_argument("--dump", default=None,
+                    help="write the mixed run's per-trajectory status, steps and handover step of "
+                         "every 4th site to this .npz file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the tracer shards by "
@@ -148,8 +151,14 @@ def main():
           file=sys.stderr, flush=True)
     t_mixed, k_mixed = time_plan(mixed, args.steps, args.warmup)
     st_m = mixed.stats()
-    _, cut_m = mixed.results(traj=False)
+    tr_m, cut_m = mixed.results(traj=args.dump is not None)
     mixed.close()
+    if args.dump:
+        per = n // len(scans)
+        keep = (np.arange(n) // per) % 4 == 0
+        np.savez_compressed(args.dump, status=tr_m["status"][keep].astype(np.int8), steps=tr_m["steps"][keep],
+                            handover=tr_m["handover"][keep], per_site=per)
+        del tr_m
     print(f"[bench] mixed: {t_mixed*1e3:.1f} ms/step, kernel {k_mixed:.1f} ms (HIP events, mean of "
           f"{args.steps}), {st_m.steps} steps, {st_m.attempts} attempts; lane fill "
           f"{st_m.lane_iters / max(64 * st_m.wave_iters, 1):.3f}, queue drained at {st_m.drain_ms:.0f} ms, "

@@ -17,7 +17,7 @@ NCOEF = 105
 DP = C.POINTER(C.c_double)
 
 COR_OK, COR_E_INVALID, COR_E_NO_DEVICE, COR_E_HIP, COR_E_UNSUPPORTED, COR_E_NOMEM = 0, -1, -2, -3, -4, -5
-PREC_FP64, PREC_MIXED, PREC_MIXED_WAVE = 0, 1, 2
+PREC_FP64, PREC_MIXED, PREC_MIXED_WAVE, PREC_MIXED_GROUP = 0, 1, 2, 3
 INTERNAL_NONE, INTERNAL_IGRF = 0, 1
 EXTERNAL_NONE, EXTERNAL_T96, EXTERNAL_TS05 = 0, 1, 2
 CRUSTAL_NONE, CRUSTAL_SH = 0, 1
@@ -82,7 +82,7 @@ class TraceStats(C.Structure):
                 ("drain_ms", C.c_double), ("lane_end_ms", C.c_double)]
 
 
-TRAJ_DTYPE = np.dtype([("status", "<i4"), ("steps", "<u4"), ("attempts", "<u4"), ("pad", "<u4"),
+TRAJ_DTYPE = np.dtype([("status", "<i4"), ("steps", "<u4"), ("attempts", "<u4"), ("handover", "<u4"),
                        ("vc", "<f8"), ("r", "<f8"), ("td", "<f8"), ("fei", "<f8"), ("ast", "<f8"),
                        ("asf", "<f8"), ("time", "<f8"), ("length_m", "<f8")])
 SCAN_DTYPE = np.dtype([("r0", "<f8"), ("the0", "<f8"), ("fi0", "<f8"), ("the1", "<f8"),

@@ -298,6 +298,26 @@ __device__ __forceinline__ void wave_sum(float *p) {
     for (int k = 0; k < N; k++) p[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[k]), 63));
 }
 
+// Sum of each of N values over a group of G lanes (G = 16: the four DPP row steps; every
+// lane of the row ends with the same bits, since each step adds two values that the
+// partner lane adds in the other order).  G = 64: wave_sum.
+template <int N, int G>
+__device__ __forceinline__ void group_sum(float *p) {
+    static_assert(G == 16 || G == 64, "groups of 16 or 64 lanes");
+    if constexpr (G == 64) {
+        wave_sum<N>(p);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) p[k] = dpp_add<0xB1, 0xF>(p[k]);    // quad_perm [1,0,3,2]
+#pragma unroll
+        for (int k = 0; k < N; k++) p[k] = dpp_add<0x4E, 0xF>(p[k]);    // quad_perm [2,3,0,1]
+#pragma unroll
+        for (int k = 0; k < N; k++) p[k] = dpp_add<0x141, 0xF>(p[k]);   // row_half_mirror
+#pragma unroll
+        for (int k = 0; k < N; k++) p[k] = dpp_add<0x140, 0xF>(p[k]);   // row_mirror
+    }
+}
+
 // This lane's rows of the lane tables, read together at the top of a step so their LDS
 // latencies overlap (rows past a family's size are valid zero or unused rows).
 struct CoopLane {
@@ -313,19 +333,24 @@ __device__ __forceinline__ void put4(float *d, float4 v) {
     d[3] = v.w;
 }
 
-__device__ __forceinline__ void load_lane(const CoopTab &T, int lane, CoopLane &L) {
-    L.harm = ld4(T.harm[lane]);
-    L.cone = ld4(T.cone[lane & 7]);
-    L.cyl = ld4(T.cyl[lane & 7]);
-    L.t87 = ld4(T.t87[lane & 3]);
-    L.conic = ld4(T.conic[lane & 15]);
-    const int ci = lane < 24 ? lane : 0;
+// gl: the lane's index in its group.  With groups of 16 the Cartesian harmonics and the
+// current loops (more terms than lanes) are read in their passes instead.
+template <int G>
+__device__ __forceinline__ void load_lane(const CoopTab &T, int gl, CoopLane &L) {
+    if constexpr (G == 64) L.harm = ld4(T.harm[gl]);
+    L.cone = ld4(T.cone[gl & 7]);
+    L.cyl = ld4(T.cyl[gl & 7]);
+    L.t87 = ld4(T.t87[gl & 3]);
+    L.conic = ld4(T.conic[gl & 15]);
 #pragma unroll
-    for (int j = 0; j < 6; j++) put4(L.icon + 4 * j, ld4(&T.icon[lane & 3][4 * j]));
+    for (int j = 0; j < 6; j++) put4(L.icon + 4 * j, ld4(&T.icon[gl & 3][4 * j]));
+    if constexpr (G == 64) {
+        const int ci = gl < 24 ? gl : 0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) put4(L.circ + 4 * j, ld4(&T.circ[ci][4 * j]));
+        for (int j = 0; j < 4; j++) put4(L.circ + 4 * j, ld4(&T.circ[ci][4 * j]));
+    }
 #pragma unroll
-    for (int j = 0; j < 6; j++) put4(L.sheet + 4 * j, ld4(&T.sheet[lane & 15][4 * j]));
+    for (int j = 0; j < 6; j++) put4(L.sheet + 4 * j, ld4(&T.sheet[gl & 15][4 * j]));
 }
 
 // One term of the harmonic form shared by the shield sets, birk1shld, birk2shl and the
@@ -350,8 +375,9 @@ struct T96Share {
     float sigma, oimfy, oimfz;
 };
 
+template <int G>
 __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, const CoopTab &T, float x, float y,
-                                                   float z, int lane) {
+                                                   float z, int gl) {
     using namespace t96;
     typedef float F;
     T96Share S;
@@ -372,7 +398,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
     S.fx = mk3<F>(0, 0, 0);
     if (S.sigma >= F(1.08 + 0.005)) return S;
     CoopLane L;
-    load_lane(T, lane, L);
+    load_lane<G>(T, gl, L);
     const F xx = x * P.xappa, yy = y * P.xappa, zz = z * P.xappa;
     F px = 0, py = 0, pz = 0;
 
@@ -457,17 +483,27 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         }
     }
 
-    // ---- family 1: Cartesian harmonics (47 lanes)
-    {
+    // ---- family 1: Cartesian harmonics (47 terms: one pass of 64 lanes, or three of 16
+    // over rows 0..47, row 47 being a zero row)
+    if constexpr (G == 64) {
         const float4 h = L.harm;
         const v3<F> b = co_harm<F>(xx, yy, zz, h.x, h.y, h.z, h.w);
         px += b.x;
         py += b.y;
         pz += b.z;
+    } else {
+#pragma unroll
+        for (int k = 0; k < (CO_NHARM + G - 1) / G; k++) {
+            const float4 h = ld4(T.harm[gl + G * k]);
+            const v3<F> b = co_harm<F>(xx, yy, zz, h.x, h.y, h.z, h.w);
+            px += b.x;
+            py += b.y;
+            pz += b.z;
+        }
     }
     // ---- family 2: interconnection harmonics (3 lanes, one y harmonic each) at the
     // IMF-rotated point, rotated back (lib/t96/t96.c:66-72)
-    if (lane < CO_NICON) {
+    if (gl < CO_NICON) {
         const float *c = L.icon;
         const F X = xx, Y = ys * P.xappa, Z = zs * P.xappa;
         F sy, cy, sz, cz, s2, c2;
@@ -490,7 +526,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         pz += bz * P.ct - by * P.st;
     }
     // ---- family 3: conical sheets of the ring current and the tail disk (6 lanes)
-    if (lane < CO_NCONE) {
+    if (gl < CO_NCONE) {
         const float4 c = L.cone;
         const F f = c.x, beta = c.y, xs = xx - c.z, d1 = c.w;
         const F y20 = yy / F(20.0);
@@ -514,7 +550,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         pz += f * ((F(2.0) * as + yy * dasy) + xs * dasx);
     }
     // ---- family 4: cylindrical harmonics (6 lanes)
-    if (lane < CO_NCYL) {
+    if (gl < CO_NCYL) {
         const float4 c = L.cyl;
         const F a = c.x, scale = c.y;
         F rho = Dm<F>::sqrt(yy * yy + zz * zz);
@@ -546,7 +582,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         }
     }
     // ---- family 5: tail87 sheet and images (3 lanes)
-    if (lane < CO_NT87) {
+    if (gl < CO_NT87) {
         const float4 c = L.t87;
         const F zl = zz + c.x, w = c.y;
         const F xnx = F(-10.0) - xx, xnx2 = xnx * xnx;
@@ -570,8 +606,8 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
     {
         const int d0 = need_loop ? 0 : CO_NDIP_LOOP;
         const int d1 = need_cone ? CO_NDIP : CO_NDIP_LOOP;
-        for (int base = d0; base < d1; base += 64) {
-            const int s = base + lane;
+        for (int base = d0; base < d1; base += G) {
+            const int s = base + gl;
             if (s < d1) {
                 const float4 dp = ld4(&T.dip[s][0]);
                 const float4 mm = ld4(&T.dip[s][4]);
@@ -587,9 +623,17 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
             }
         }
     }
-    // ---- family 7: current loops (18 lanes; groups weighted by wl, wo, wi)
-    if (lane < CO_NCIRC) {
-        const float *c = L.circ;
+    // ---- family 7: current loops (18 terms; groups weighted by wl, wo, wi)
+#pragma unroll
+    for (int k = 0; k < (CO_NCIRC + G - 1) / G; k++) {
+        const int t = gl + G * k;
+        if (t >= CO_NCIRC) break;
+        float cbuf[16];
+        if constexpr (G != 64) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) put4(cbuf + 4 * j, ld4(&T.circ[t][4 * j]));
+        }
+        const float *c = G == 64 ? L.circ : cbuf;
         const int g = (int)c[14];
         const F w = g == 0 ? wl : (g == 1 ? wo : wi);
         if (w != F(0.0)) {
@@ -605,7 +649,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         }
     }
     // ---- family 8: conical harmonics (10 lanes: Region-1 conical basis, Region-2 inner)
-    if (lane < CO_NCONIC) {
+    if (gl < CO_NCONIC) {
         const float4 c = L.conic;
         const bool g0 = c.w == F(0.0);
         const F w = g0 ? wc : wi;
@@ -643,7 +687,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         }
     }
     // ---- family 9: Region-2 sheet (15 lanes)
-    if (ws != F(0.0) && lane < CO_NSHEET) {
+    if (ws != F(0.0) && gl < CO_NSHEET) {
         const float *c = L.sheet;
         const int comp = (int)c[3];
         F t1, t2, t3;
@@ -670,7 +714,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         else pz += v;
     }
     // ---- Region-2 inner dipole distributions (uniform, counted on lane 0)
-    if (wi != F(0.0) && lane == 0) {
+    if (wi != F(0.0) && gl == 0) {
         const v3<F> d6 = dipdistr(xx - F(.0774), yy, zz, 0);
         const v3<F> d7 = dipdistr(xx + F(.038), yy, zz, 1);
         const F k = wi * P.b2ampl;
@@ -707,7 +751,7 @@ struct IgrfShare {
 
 __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G, const float *__restrict__ H,
                                                      const float *__restrict__ REC, float x, float y, float z,
-                                                     int lane) {
+                                                     int gl) {
     typedef float F;
     IgrfShare S;
     const F rho2 = x * x + y * y;
@@ -726,7 +770,7 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
     int nm = 3 + 30 / ((int)r + 2);
     if (nm > 13) nm = 13;
     const int kw = nm + 1;
-    const int m = lane + 1;
+    const int m = gl + 1;     // groups of at least 16 lanes: one order each (at most 14)
     F am = inv_r * inv_r, pmm = F(1.), dmm = F(0.), cm = F(0.), sm = F(1.);
     for (int j = 1; j < kw; j++) {
         if (j < m) {
@@ -784,28 +828,30 @@ __device__ __forceinline__ v3<float> igrf_finish(const IgrfShare &S, float br, f
 // ---------------------------------------------------------------- total field
 
 // GEO field [nT] of the model set at the GEO position geo_m / GSM position gsm_m [m],
-// identical in every lane of the wavefront.
-template <int INT, int EXT>
+// identical in every lane of a group of GS lanes (gl: the lane's index in the group;
+// each group may be at its own position).
+template <int INT, int EXT, int GS>
 __device__ __forceinline__ v3<float> coop_field(const float *G, const float *H, const float *REC, const float *Af,
                                                 const T96Params<float> &P, const CoopTab &T, v3<double> geo_m,
-                                                v3<double> gsm_m, int lane) {
+                                                v3<double> gsm_m, int gl) {
+    static_assert(GS >= 16, "one IGRF order per lane needs 14 lanes");
     typedef float F;
     IgrfShare I;
     T96Share X;
     F p[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (INT == INT_IGRF) {
-        I = coop_igrf_share(G, H, REC, (F)(geo_m.x / K_RE), (F)(geo_m.y / K_RE), (F)(geo_m.z / K_RE), lane);
+        I = coop_igrf_share(G, H, REC, (F)(geo_m.x / K_RE), (F)(geo_m.y / K_RE), (F)(geo_m.z / K_RE), gl);
         p[0] = I.br;
         p[1] = I.bt;
         p[2] = I.bf;
     }
     if constexpr (EXT == EXT_T96) {
-        X = coop_t96_share(P, T, (F)(gsm_m.x / K_RE), (F)(gsm_m.y / K_RE), (F)(gsm_m.z / K_RE), lane);
+        X = coop_t96_share<GS>(P, T, (F)(gsm_m.x / K_RE), (F)(gsm_m.y / K_RE), (F)(gsm_m.z / K_RE), gl);
         p[3] = X.fx.x;
         p[4] = X.fx.y;
         p[5] = X.fx.z;
     }
-    wave_sum<6>(p);
+    group_sum<6, GS>(p);
     v3<F> tot = mk3<F>(0, 0, 0);
     if constexpr (INT == INT_IGRF) tot = igrf_finish(I, p[0], p[1], p[2]);
     if constexpr (EXT == EXT_T96) {

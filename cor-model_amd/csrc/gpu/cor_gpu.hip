@@ -228,12 +228,19 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
         coop->res = 0ull;
         coop->head = 0ull;
         coop->running = 0;
+        coop->reparks = 0ull;
+        coop->done = 0ull;
     }
 }
 
-// once the queue is drained, wavefronts with at most this many live trajectories hand
-// them to the cooperative mode (fuller ones do so when enough workers are idle)
-constexpr int COOP_LANES_DEFAULT = 4;
+// Once the queue is drained, a wavefront with at most COOP_LANES_DEFAULT live
+// trajectories hands them to the cooperative mode, and a fuller one does so when
+// waiting workers times COOP_IDLE_WEIGHT reach its live count: one waiting worker starts
+// taking a wavefront's trajectories, the rest are claimed four to a worker from the
+// parked pool.  Both measured on the 5-degree IGRF + T96 grid (full grid: 21.8 s with
+// 4 and 1, 17.5 s with 32 and 64).
+constexpr int COOP_LANES_DEFAULT = 32;
+constexpr long long COOP_IDLE_WEIGHT = 64;
 
 // the mixed T96 evaluator reads its interconnection series from the lane tables
 bool needs_coop_tables(int precision, int ext) { return precision != COR_PREC_FP64 && ext == EXT_T96; }
@@ -337,6 +344,7 @@ void cor_trace_plan_destroy(cor_trace_plan *p) {
     (void)hipFree(p->out.status);
     (void)hipFree(p->out.steps);
     (void)hipFree(p->out.attempts);
+    (void)hipFree(p->out.handover);
     (void)hipFree(p->out.vc);
     (void)hipFree(p->d_cut);
     (void)hipFree(p->d_stat);
@@ -413,6 +421,7 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     PLAN_TRY(hipMalloc(&p->out.status, sizeof(int32_t) * nt));
     PLAN_TRY(hipMalloc(&p->out.steps, sizeof(uint32_t) * nt));
     PLAN_TRY(hipMalloc(&p->out.attempts, sizeof(uint32_t) * nt));
+    PLAN_TRY(hipMalloc(&p->out.handover, sizeof(uint32_t) * nt));
     // the eight per-trajectory doubles share one allocation, one array after another
     PLAN_TRY(hipMalloc(&p->out.vc, sizeof(double) * nt * 8));
     p->out.r = p->out.vc + nt;
@@ -470,7 +479,9 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     if (p->coop_lanes > 0) {
         // cooperative tail: the handover counters and one park slot per lane of the
         // grid (a lane parks at most one trajectory per run)
-        const size_t slots = (size_t)p->grid * p->block;
+        // one park slot per lane of the grid for the per-lane mode's handovers, as many
+        // again for trajectories that cooperative workers hand back
+        const size_t slots = 2 * (size_t)p->grid * p->block;
         PLAN_TRY(hipMalloc(&p->d_coop, sizeof(CoopCtl)));
         PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * slots));
         PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * slots, g_stream));
@@ -506,6 +517,8 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.coop = p->d_coop;
     a.park = p->d_park;
     a.stats = p->d_run;
+    a.repark_limit = (long long)p->grid * p->block;
+    a.idle_weight = COOP_IDLE_WEIGHT;
     reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop, p->d_run);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(p->ev[0], s));
@@ -531,15 +544,26 @@ int cor_trace_plan_sync(cor_trace_plan *p) {
 int cor_trace_plan_results(cor_trace_plan *p, cor_traj *traj, cor_cutoff *cut) {
     if (!p || !p->ran) return COR_E_INVALID;
     HIP_TRY(hipEventSynchronize(p->ev[2]));
+    if (p->coop_lanes > 0) {
+        // every trajectory is counted once when it ends, in either mode
+        CoopCtl c;
+        HIP_TRY(hipMemcpy(&c, p->d_coop, sizeof c, hipMemcpyDeviceToHost));
+        if (c.done != (unsigned long long)p->n_items) {
+            snprintf(g_hip_error, sizeof g_hip_error, "cooperative tail finished %llu of %lld trajectories",
+                     c.done, (long long)p->n_items);
+            return COR_E_HIP;
+        }
+    }
     if (cut) HIP_TRY(hipMemcpy(cut, p->d_cut, sizeof(cor_cutoff) * p->n_scans, hipMemcpyDeviceToHost));
     if (traj && p->n_items > 0) {
         const size_t n = (size_t)p->n_items;
         std::vector<int32_t> st(n);
-        std::vector<uint32_t> sp(n), at(n);
+        std::vector<uint32_t> sp(n), at(n), ho(n);
         std::vector<double> d(n * 8);
         HIP_TRY(hipMemcpy(st.data(), p->out.status, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(sp.data(), p->out.steps, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(at.data(), p->out.attempts, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ho.data(), p->out.handover, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         // the eight arrays were allocated as one block of n_items-strided arrays
         const size_t nt = (size_t)std::max<int64_t>(p->n_items, 1);
         std::vector<double> blk(nt * 8);
@@ -549,7 +573,7 @@ int cor_trace_plan_results(cor_trace_plan *p, cor_traj *traj, cor_cutoff *cut) {
             t.status = st[i];
             t.steps = sp[i];
             t.attempts = at[i];
-            t.pad = 0;
+            t.handover = ho[i];
             t.vc = blk[i];
             t.r = blk[nt + i];
             t.td = blk[2 * nt + i];
@@ -678,15 +702,18 @@ int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const 
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
                   double *geo_nT, int64_t n, int32_t precision) {
     if (!epoch || n < 0 || (n > 0 && (!gsm_m || !geo_nT))) return COR_E_INVALID;
-    if (precision != COR_PREC_FP64 && precision != COR_PREC_MIXED && precision != COR_PREC_MIXED_WAVE)
+    if (precision != COR_PREC_FP64 && precision != COR_PREC_MIXED && precision != COR_PREC_MIXED_WAVE &&
+        precision != COR_PREC_MIXED_GROUP)
         return COR_E_INVALID;
+    const bool coop = precision == COR_PREC_MIXED_WAVE || precision == COR_PREC_MIXED_GROUP;
+    const int lanes = precision == COR_PREC_MIXED_GROUP ? 16 : 64;
     int rc = ensure_init();
     if (rc) return rc;
     int intk, ext, cru;
     if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
     const void *kern = precision == COR_PREC_FP64    ? field_kernel_fp64(intk, ext, cru)
                        : precision == COR_PREC_MIXED ? field_kernel_mixed(intk, ext, cru)
-                                                     : field_kernel_coop(intk, ext, cru);
+                                                     : field_kernel_coop(intk, ext, cru, lanes);
     if (!kern) return COR_E_UNSUPPORTED;
     if (n == 0) return COR_OK;
     EpochDev E;
@@ -698,7 +725,7 @@ int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const 
     if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 3 * n);
     if (e == hipSuccess) e = hipMemcpyAsync(d_e, &E, sizeof(EpochDev), hipMemcpyHostToDevice, g_stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_in, gsm_m, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g_stream);
-    if (e == hipSuccess && (precision == COR_PREC_MIXED_WAVE || needs_coop_tables(precision, ext)))
+    if (e == hipSuccess && (coop || needs_coop_tables(precision, ext)))
         e = build_coop_tables(d_e, g_stream);
     if (e == hipSuccess) {
         FieldBatchArgs a;
@@ -707,9 +734,9 @@ int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const 
         a.geo_nT = d_out;
         a.n = n;
         void *args[] = {&a};
-        // one lane per point, or one wavefront per point (4 per block, strided)
-        const int64_t blocks = precision == COR_PREC_MIXED_WAVE ? std::min<int64_t>((n + 3) / 4, 4096)
-                                                                : (n + 255) / 256;
+        // one lane per point, or one cooperative group per point (strided)
+        const int64_t per_block = 256 / lanes;
+        const int64_t blocks = coop ? std::min<int64_t>((n + per_block - 1) / per_block, 4096) : (n + 255) / 256;
         e = hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(256), args, 0, g_stream);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(geo_nT, d_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, g_stream);

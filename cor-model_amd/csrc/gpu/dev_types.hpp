@@ -86,15 +86,19 @@ struct ParkDev {
     int64_t gid;
     uint32_t step, attempts;
     int32_t nk1;
+    uint32_t handover;           // 1 + step of the first handover (0: this one is the first)
     uint32_t ready;              // = run id once the record is complete
 };
 
 // Work-handover counters of one run (cleared by the reset kernel).
+// The counters each sit in a cache line of their own: workers poll them while others
+// update them.
 struct CoopCtl {
-    unsigned long long res;      // park slots reserved
-    unsigned long long head;     // tickets taken by cooperative workers
-    int32_t running;             // wavefronts still in the per-lane mode
-    int32_t pad;
+    alignas(128) unsigned long long res;      // park slots reserved
+    alignas(128) unsigned long long head;     // tickets taken by cooperative workers
+    alignas(128) int32_t running;             // wavefronts still in the per-lane mode
+    alignas(128) unsigned long long reparks;  // trajectories handed back by cooperative workers
+    alignas(128) unsigned long long done;     // trajectories finished (both modes)
 };
 
 struct ScanDev {
@@ -109,6 +113,7 @@ struct TraceOut {
     int32_t *status;
     uint32_t *steps;
     uint32_t *attempts;
+    uint32_t *handover;          // 1 + the step of the handover to the cooperative mode, or 0
     double *vc, *r, *td, *fei, *ast, *asf, *time, *length;
 };
 
@@ -147,6 +152,8 @@ struct TraceArgs {
     CoopCtl *coop;
     ParkDev *park;
     RunStats *stats;
+    long long repark_limit;      // park slots beyond the per-lane ones (one per lane of the grid)
+    long long idle_weight;       // trajectories one waiting worker is counted to take
 };
 
 }  // namespace cor

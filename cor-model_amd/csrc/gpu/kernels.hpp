@@ -24,7 +24,7 @@ const void *field_kernel_mixed(int intk, int ext, int cru);
 // Mixed-precision field batch evaluated like the tracer's cooperative mode (one
 // wavefront per point), and the one-thread kernel that builds the cooperative lane
 // tables of an epoch in place (EpochDev::coop).
-const void *field_kernel_coop(int intk, int ext, int cru);
+const void *field_kernel_coop(int intk, int ext, int cru, int lanes);
 const void *coop_build_kernel();
 
 }  // namespace cor

@@ -28,21 +28,28 @@ __global__ void __launch_bounds__(256) field_batch_kernel(FieldBatchArgs a) {
     a.geo_nT[3 * i + 2] = out.z;
 }
 
-// The same batch with the cooperative evaluator: one wavefront per point, points
-// strided over the grid's wavefronts.
-template <int INT, int EXT, int CRU>
+// The same batch with the cooperative evaluator as the tracer's cooperative mode runs
+// it: one group of GS lanes (16 or 64) per point, points strided over the grid's groups.
+// Every lane of a wavefront takes part in each pass (the group sums need all lanes);
+// groups past the end evaluate the last point again and write nothing.
+template <int INT, int EXT, int CRU, int GS>
 __global__ void __launch_bounds__(256) field_coop_kernel(FieldBatchArgs a) {
     __shared__ float s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
     __shared__ CoopTab s_coop;
     const EpochView<float> E = stage_epoch<float>(a.epoch, s_g, s_h, s_rec);
     stage_coop(&a.epoch->coop, &s_coop);
-    const int lane = __lane_id();
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
-    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; i < a.n; i += waves) {
-        const v3<double> gsm = mk3(a.gsm_m[3 * i], a.gsm_m[3 * i + 1], a.gsm_m[3 * i + 2]);
+    const int gl = __lane_id() & (GS - 1);
+    const int64_t groups = (int64_t)gridDim.x * (blockDim.x / GS);
+    const int64_t first = (int64_t)blockIdx.x * (blockDim.x / GS) + threadIdx.x / GS;
+    // the wavefront's groups loop together: as many rounds as its first group needs
+    const int64_t wave_first = first - (int64_t)((threadIdx.x & 63) / GS);
+    for (int64_t base = wave_first; base < a.n; base += groups) {
+        const int64_t i = base + (first - wave_first);
+        const int64_t ic = i < a.n ? i : a.n - 1;
+        const v3<double> gsm = mk3(a.gsm_m[3 * ic], a.gsm_m[3 * ic + 1], a.gsm_m[3 * ic + 2]);
         const v3<double> geo = rot_gsm_to_geo(E.A, gsm);
-        const v3<float> f = coop_field<INT, EXT>(E.G, E.H, E.REC, E.Af, E.t96, s_coop, geo, gsm, lane);
-        if (lane == 0) {
+        const v3<float> f = coop_field<INT, EXT, GS>(E.G, E.H, E.REC, E.Af, E.t96, s_coop, geo, gsm, gl);
+        if (gl == 0 && i < a.n) {
             a.geo_nT[3 * i] = f.x;
             a.geo_nT[3 * i + 1] = f.y;
             a.geo_nT[3 * i + 2] = f.z;
@@ -79,10 +86,10 @@ const void *field_kernel_table(int intk, int ext, int cru) {
     return nullptr;
 }
 
-template <int PREC>
+template <int GS>
 const void *field_coop_table(int intk, int ext, int cru) {
 #define COR_PICK(I, E, C) \
-    if (intk == I && ext == E && cru == C) return (const void *)&field_coop_kernel<I, E, C>;
+    if (intk == I && ext == E && cru == C) return (const void *)&field_coop_kernel<I, E, C, GS>;
     COR_COOP_MODEL_SETS(COR_PICK)
 #undef COR_PICK
     return nullptr;

@@ -9,8 +9,8 @@ const void *trace_kernel_mixed(int intk, int ext, int cru) {
 const void *field_kernel_mixed(int intk, int ext, int cru) {
     return field_kernel_table<PREC_MIXED>(intk, ext, cru);
 }
-const void *field_kernel_coop(int intk, int ext, int cru) {
-    return field_coop_table<PREC_MIXED>(intk, ext, cru);
+const void *field_kernel_coop(int intk, int ext, int cru, int lanes) {
+    return lanes == 16 ? field_coop_table<16>(intk, ext, cru) : field_coop_table<64>(intk, ext, cru);
 }
 
 __global__ void build_coop_kernel(EpochDev *E) {

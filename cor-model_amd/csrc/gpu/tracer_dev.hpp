@@ -303,10 +303,11 @@ __device__ __forceinline__ int find_scan(const ScanDev *__restrict__ scans, int 
 // zeroed, so every run of a plan writes every output (a re-run at another epoch leaves
 // nothing of the previous one).
 __device__ __forceinline__ void write_end(const TraceArgs &a, int64_t gid, int status, unsigned int step,
-                                          uint32_t attempts) {
+                                          uint32_t attempts, uint32_t handover = 0) {
     a.out.status[gid] = status;
     a.out.steps[gid] = step;
     a.out.attempts[gid] = attempts;
+    a.out.handover[gid] = handover;
     if (status != ST_ALLOWED) {
         a.out.vc[gid] = 0.;
         a.out.r[gid] = 0.;
@@ -503,6 +504,13 @@ __device__ __forceinline__ unsigned long long load_acquire(unsigned long long *p
 __device__ __forceinline__ int load_acquire(int32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// relaxed load of a shared counter, made wavefront-uniform
+__device__ __forceinline__ unsigned long long load_relaxed_uniform(unsigned long long *p) {
+    const unsigned long long x = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)x);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(x >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
 
 // Hand this wavefront's live trajectories to the cooperative workers (called with the
 // whole wavefront active, at the top of a step): lane 0 reserves one park slot per live
@@ -532,61 +540,197 @@ __device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool ac
         P->step = step;
         P->attempts = attempts;
         P->nk1 = nk1;
+        P->handover = 0u;
         __hip_atomic_store(&P->ready, a.run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// Cooperative worker: takes tickets for park slots in order and runs each parked
-// trajectory to its end with the whole wavefront.  A ticket beyond the reserved slots
-// waits until a slot appears or no wavefront is left in the per-lane mode (then the
-// reserved count is final and the worker retires).  Never waits for another worker,
-// and parkers never wait at all, so every wait ends.
+// Cooperative worker.  A wavefront runs up to four parked trajectories, one per group of
+// 16 lanes, while parked work is plentiful, and a lone trajectory with all 64 lanes
+// (its step then takes about a third of the time); groups of a wavefront hold the same
+// state in all their lanes and compute the same bits.
+//   * An idle wavefront takes one ticket for the next park slot (tickets are served in
+//     slot order) and waits until the slot is reserved and published.
+//   * A busy wavefront takes tickets for its free groups only for slots already
+//     reserved and not yet claimed (compare-and-swap on the ticket counter).
+//   * A wavefront running two or more trajectories while another waits hands one back
+//     (re-park, at most one park slot per lane of the grid in all), so the trajectories
+//     spread one per wavefront as the work thins out.
+// An idle wavefront keeps its ticket until every trajectory of the run has finished
+// (a counter both modes add to), so a trajectory handed back is always taken; then it
+// retires.  Every slot is served by the holder of its ticket, so the count is reached;
+// as a guard against a lost record, an idle wavefront also retires after 60 s without
+// any trajectory finishing once no wavefront is left in the per-lane mode (the host
+// then reports the shortfall).  Workers never wait for one another and parkers never
+// wait.
 template <int INT, int EXT>
 __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const CoopTab &T, int lane) {
+    constexpr int GS = 16;
+    const int gl = lane & (GS - 1);
+    const int lead = lane & ~(GS - 1);            // first lane of this lane's group
+    const bool is_lead = gl == 0;
+    bool has = false, holding = false;
+    unsigned long long ticket = 0;
+    v3<double> pos = mk3(2.0 * K_RE, 1.0 * K_RE, 0.5 * K_RE), vel = mk3(0., 0., 0.);
+    double v = 0., time = 0., alength = 0., zn = 0.;
+    int64_t gid = 0;
+    unsigned int step = 0;
+    uint32_t attempts = 0, handover = 0;
+    int nk1 = 1, nk0 = 1;
+    unsigned long long evals = 0;
+    bool repark_ok = true;
+    unsigned iter = 0;
+    unsigned long long done_seen = ~0ull, idle_since = 0;
     for (;;) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(&a.coop->head, 1ull);
-        t = __shfl(t, 0);
-        bool got;
-        for (;;) {
-            if (t < load_acquire(&a.coop->res)) {
-                got = true;
-                break;
+        iter++;
+        uint64_t has_leads = __ballot(has && is_lead);
+        // ---- 1. tickets and the start of parked trajectories (counters read relaxed:
+        // an acquire load would invalidate the caches of the model constants; acquire
+        // is used where it orders data: a published slot, and running == 0 before res
+        // is taken as final)
+        // a busy wavefront looks at the shared counters every 16th step only
+        const bool look = !has_leads || (iter & 15u) == 0;
+        if (look && __popcll(has_leads) < 64 / GS) {
+            const bool free_g = !has && !holding;
+            const uint64_t free_leads = __ballot(free_g && is_lead);
+            const unsigned long long head0 = load_relaxed_uniform(&a.coop->head);
+            const unsigned long long res0 = load_relaxed_uniform(&a.coop->res);
+            int want = 0;
+            bool wait_ok = false;
+            if (!has_leads && !__any(holding)) {
+                want = 1;
+                wait_ok = true;
+            } else if (free_leads && head0 < res0) {
+                want = (int)min((unsigned long long)__popcll(free_leads), res0 - head0);
             }
-            if (load_acquire(&a.coop->running) == 0) {
-                got = t < load_acquire(&a.coop->res);
-                break;
+            if (want > 0) {
+                const int first = __ffsll((unsigned long long)free_leads) - 1;
+                unsigned long long base = ~0ull;
+                if (lane == first) {
+                    if (wait_ok) {
+                        base = atomicAdd(&a.coop->head, (unsigned long long)want);
+                    } else if (atomicCAS(&a.coop->head, head0, head0 + (unsigned long long)want) == head0) {
+                        base = head0;
+                    }
+                }
+                base = __shfl(base, first);
+                if (base != ~0ull && free_g) {
+                    const int rank = __popcll(free_leads & ((1ull << lead) - 1ull));
+                    if (rank < want) {
+                        ticket = base + (unsigned long long)rank;
+                        holding = true;
+                    }
+                }
             }
-            __builtin_amdgcn_s_sleep(32);
+            int running = __hip_atomic_load(&a.coop->running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned long long res;
+            if (running == 0) {
+                running = load_acquire(&a.coop->running);
+                res = load_acquire(&a.coop->res);
+            } else {
+                res = __hip_atomic_load(&a.coop->res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (holding && ticket < res) {
+                ParkDev *P = a.park + ticket;
+                if (__hip_atomic_load(&P->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == a.run_id) {
+                    pos = mk3(P->pos[0], P->pos[1], P->pos[2]);
+                    vel = mk3(P->vel[0], P->vel[1], P->vel[2]);
+                    v = P->v;
+                    time = P->time;
+                    alength = P->alength;
+                    zn = P->zn;
+                    gid = P->gid;
+                    step = P->step;
+                    attempts = P->attempts;
+                    nk1 = P->nk1;
+                    nk0 = nk1;
+                    handover = P->handover ? P->handover : step + 1u;
+                    evals = 0;
+                    has = true;
+                    holding = false;
+                }
+            }
+            has_leads = __ballot(has && is_lead);
+            if (!has_leads) {
+                const unsigned long long done = load_relaxed_uniform(&a.coop->done);
+                if (done >= (unsigned long long)a.n_items) return;
+                const unsigned long long now = wall_clock64();
+                if (done != done_seen || running != 0) {
+                    done_seen = done;
+                    idle_since = now;
+                } else if (now - idle_since > 60ull * 100000000ull) {
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(127);
+                continue;
+            }
         }
-        if (!got) return;
-        ParkDev *P = a.park + t;
-        while (__hip_atomic_load(&P->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.run_id)
-            __builtin_amdgcn_s_sleep(2);
-        v3<double> pos = mk3(P->pos[0], P->pos[1], P->pos[2]);
-        v3<double> vel = mk3(P->vel[0], P->vel[1], P->vel[2]);
-        double v = P->v, time = P->time, alength = P->alength;
-        const double zn = P->zn;
-        const int64_t gid = P->gid;
-        unsigned int step = P->step;
-        uint32_t attempts = P->attempts;
-        const int nk1 = P->nk1;
-        int nk0 = nk1;
-        unsigned long long evals = 0;
-        for (;;) {
-            evals++;
+        // ---- 2. spread: with two or more trajectories here and a wavefront waiting for
+        // work, hand the last group's trajectory back
+        if (look && __popcll(has_leads) >= 2 && repark_ok) {
+            const unsigned long long head1 = load_relaxed_uniform(&a.coop->head);
+            const unsigned long long res1 = load_relaxed_uniform(&a.coop->res);
+            if (head1 > res1) {
+                const int src = 63 - __clzll((long long)has_leads);    // lead lane of the last busy group
+                unsigned long long r = 0;
+                if (lane == src) r = atomicAdd(&a.coop->reparks, 1ull);
+                r = __shfl(r, src);
+                if (r < (unsigned long long)a.repark_limit) {
+                    if (lead == src) {
+                        if (is_lead) {
+                            const unsigned long long slot = atomicAdd(&a.coop->res, 1ull);
+                            ParkDev *P = a.park + slot;
+                            P->pos[0] = pos.x;
+                            P->pos[1] = pos.y;
+                            P->pos[2] = pos.z;
+                            P->vel[0] = vel.x;
+                            P->vel[1] = vel.y;
+                            P->vel[2] = vel.z;
+                            P->v = v;
+                            P->time = time;
+                            P->alength = alength;
+                            P->zn = zn;
+                            P->gid = gid;
+                            P->step = step;
+                            P->attempts = attempts;
+                            P->nk1 = nk1;
+                            P->handover = handover;
+                            if (evals) atomicAdd(&a.stats->coop_steps, evals);
+                            __hip_atomic_store(&P->ready, a.run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        has = false;
+                    }
+                    has_leads = __ballot(has && is_lead);
+                } else {
+                    repark_ok = false;
+                }
+            }
+        }
+        // ---- 3. one step of each trajectory: a lone one with all 64 lanes, several with
+        // a group of 16 lanes each (groups without one evaluate at their last point)
+        v3<float> f;
+        if (__popcll(has_leads) == 1) {
+            const int src = __ffsll((unsigned long long)has_leads) - 1;
+            const v3<double> p1 = mk3(__shfl(pos.x, src), __shfl(pos.y, src), __shfl(pos.z, src));
+            const v3<double> gsm = rot_geo_to_gsm(E.A, p1);
+            f = coop_field<INT, EXT, 64>(E.G, E.H, E.REC, E.Af, E.t96, T, p1, gsm, lane);
+        } else {
             const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
-            const v3<float> f = coop_field<INT, EXT>(E.G, E.H, E.REC, E.Af, E.t96, T, pos, gsm, lane);
+            f = coop_field<INT, EXT, GS>(E.G, E.H, E.REC, E.Af, E.t96, T, pos, gsm, gl);
+        }
+        if (has) {
+            evals++;
             const double fx = f.x, fy = f.y, fz = f.z;
             const double b = 1.E-09 * sqrt(fx * fx + fy * fy + fz * fz);
             const int status = attempts_mixed(a, E, f, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength,
-                                              gid, lane == 0);
+                                              gid, is_lead);
             if (status >= 0) {
-                if (lane == 0) {
-                    write_end(a, gid, status, step, attempts);
+                if (is_lead) {
+                    write_end(a, gid, status, step, attempts, handover);
                     atomicAdd(&a.stats->coop_steps, evals);
+                    atomicAdd(&a.coop->done, 1ull);
                 }
-                break;
+                has = false;
             }
         }
     }
@@ -636,6 +780,7 @@ trace_kernel(TraceArgs a) {
     long long idle_seen = 0;
     // fill of this wavefront's per-lane loop
     unsigned long long wave_iters = 0, lane_iters = 0, wave_iters_open = 0, lane_iters_open = 0;
+    unsigned long long ended_here = 0;
     if (lane == 0) atomicMin(&a.stats->t_start, (unsigned long long)wall_clock64());
 
     for (;;) {
@@ -689,7 +834,7 @@ trace_kernel(TraceArgs a) {
                 if (queue_open && head_seen >= (unsigned long long)a.n_items) queue_open = false;
                 if (!queue_open) {
                     const int live = __popcll(__ballot(active));
-                    if (live <= a.coop_lanes || idle_seen >= (long long)live) {
+                    if (live <= a.coop_lanes || idle_seen * a.idle_weight >= (long long)live) {
                         park_lanes(a, lane, active, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
                         break;
                     }
@@ -709,6 +854,7 @@ trace_kernel(TraceArgs a) {
             lane_iters_open += __popcll(live_mask);
         }
 
+        bool ended_now = false;
         if (active) {
             // ---- 2. field at the current point (src/trajectory.c:37-38,147-148); the
             // GSM position is recomputed from the GEO state (same arithmetic as the
@@ -735,8 +881,12 @@ trace_kernel(TraceArgs a) {
             if (status >= 0) {
                 write_end(a, gid, status, step, attempts);
                 active = false;
+                ended_now = true;
             }
         }
+        // trajectories ended here, added to the run's count when the wavefront leaves
+        // the per-lane mode
+        ended_here += __popcll(__ballot(ended_now));
     }
     if (lane == 0) {
         atomicAdd(&a.stats->wave_iters, wave_iters);
@@ -747,7 +897,10 @@ trace_kernel(TraceArgs a) {
     }
     if constexpr (kCoop) {
         if (a.coop_lanes > 0) {
-            if (lane == 0) __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) {
+                if (ended_here) atomicAdd(&a.coop->done, ended_here);
+                __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
             coop_worker<INT, EXT>(a, E, *ctab, lane);
         }
     }

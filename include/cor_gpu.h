@@ -33,9 +33,10 @@ enum {
     COR_E_NOMEM = -5
 };
 
-/* COR_PREC_MIXED_WAVE: cor_gpu_field only -- the mixed-precision model evaluated one
- * wavefront per point, as the tracer's cooperative mode evaluates it. */
-enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1, COR_PREC_MIXED_WAVE = 2 };
+/* COR_PREC_MIXED_WAVE / COR_PREC_MIXED_GROUP: cor_gpu_field only -- the mixed-precision
+ * model evaluated as the tracer's cooperative mode evaluates it, one wavefront (64 lanes)
+ * or one group of 16 lanes per point. */
+enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1, COR_PREC_MIXED_WAVE = 2, COR_PREC_MIXED_GROUP = 3 };
 enum { COR_INTERNAL_NONE = 0, COR_INTERNAL_IGRF = 1 };
 enum { COR_EXTERNAL_NONE = 0, COR_EXTERNAL_T96 = 1, COR_EXTERNAL_TS05 = 2 };
 enum { COR_CRUSTAL_NONE = 0, COR_CRUSTAL_SH = 1 };
@@ -76,9 +77,9 @@ typedef struct {
     int32_t queue_order;      /* 0 = rigidity order, 1 = longest-expected first */
     int32_t tail_lanes;       /* COR_PREC_MIXED only: once the queue is drained, a wavefront
                                * with at most this many live trajectories hands them to the
-                               * cooperative mode (one wavefront per trajectory); 0 = the
-                               * library default (4), negative = per-lane mode only; fuller
-                               * wavefronts also hand over when enough workers are idle */
+                               * cooperative mode (a group of lanes per trajectory); 0 = the
+                               * library default (32), negative = per-lane mode only; fuller
+                               * wavefronts also hand over when a worker waits for work */
     int32_t pad;
 } cor_trace_opts;
 
@@ -89,7 +90,8 @@ typedef struct {
     int32_t status;           /* COR_ALLOWED / COR_FORBIDDEN_* */
     uint32_t steps;           /* accepted integration steps */
     uint32_t attempts;        /* RK attempts including rejected ones */
-    uint32_t pad;
+    uint32_t handover;        /* mixed precision: 1 + the step at which the trajectory moved to
+                               * the cooperative mode; 0 if it never did */
     double vc, r, td, fei, ast, asf, time, length_m;
 } cor_traj;
 

@@ -80,21 +80,25 @@ def test_field_mixed_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
     assert np.quantile(err, 0.99) < 1e-4
 
 
+@pytest.mark.parametrize("lanes", [64, 16])
 @pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
-def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
-    """The tracer's cooperative evaluator (one wavefront per point, one term family per
-    SIMD pass, one cross-lane sum): same FP32 bar as the per-lane mixed path."""
+def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts, lanes):
+    """The tracer's cooperative evaluator (one wavefront, or one group of 16 lanes, per
+    point; term families in SIMD passes; one cross-lane sum): same FP32 bar as the
+    per-lane mixed path.  With groups of 16, the four points of a wavefront are
+    different points, so the groups follow different model branches."""
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
     _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
     if parmod:
         for k, v in enumerate(parmod):
             po.PARMOD[k] = v
-    got = cor.field(cor.model_set(**kinds), p, pts, cor.PREC_MIXED_WAVE)
+    prec = cor.PREC_MIXED_WAVE if lanes == 64 else cor.PREC_MIXED_GROUP
+    got = cor.field(cor.model_set(**kinds), p, pts, prec)
     want = oracle.field(L.models(**kinds), po, pts)
     r = np.linalg.norm(pts, axis=1) / RE
     scale = np.linalg.norm(want, axis=1) + 30574.0 / r ** 3
     err = np.linalg.norm(got - want, axis=1) / scale
-    print(name, "wave max rel err", err.max(), "p99", np.quantile(err, 0.99))
+    print(name, lanes, "wave max rel err", err.max(), "p99", np.quantile(err, 0.99))
     assert np.all(np.isfinite(got))
     assert err.max() < 1e-3
     assert np.quantile(err, 0.99) < 1e-4

@@ -108,7 +108,9 @@ def test_set_epoch_equals_fresh_plan(cor, precision, tail):
 def test_set_epoch_cooperative_tables_follow_the_epoch(cor):
     """Mixed precision with the cooperative tail: its lane tables are rebuilt from the
     new epoch (T96 drivers changed), so a re-pointed plan agrees with a fresh one as
-    two cooperative runs of the same epoch agree (|dRc| within half the penumbra)."""
+    two cooperative runs of the same epoch agree (|dRc| within half the penumbra, at
+    least 0.1 GV), and both agree with FP64 at the mixed trace bar (at least 0.25 GV:
+    five rigidity steps of this scan)."""
     rows = site_rows(cor, SITES, rig=0.5, dl=0.05, nk1=100)
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
     quiet = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
@@ -125,5 +127,7 @@ def test_set_epoch_cooperative_tables_follow_the_epoch(cor):
     _, c64 = cor.trace(ms, storm, rows, step_limit=200000)
     for a, b, d in zip(cg, cf, c64):
         tol = max(0.1, 0.5 * (d["ru"] - d["rl"]))
-        print("re-pointed", a["rc"], "fresh", b["rc"], "fp64", d["rc"], "tol", tol)
-        assert abs(a["rc"] - d["rc"]) <= tol + 1e-9 and abs(b["rc"] - d["rc"]) <= tol + 1e-9
+        tol64 = max(0.25, 0.5 * (d["ru"] - d["rl"]))
+        print("re-pointed", a["rc"], "fresh", b["rc"], "fp64", d["rc"], "tol", tol, tol64)
+        assert abs(a["rc"] - b["rc"]) <= tol + 1e-9
+        assert abs(a["rc"] - d["rc"]) <= tol64 + 1e-9 and abs(b["rc"] - d["rc"]) <= tol64 + 1e-9
