@@ -110,6 +110,41 @@ def copy_calibration(gib=1.0, reps=3):
           file=sys.stderr, flush=True)
 
 
+def cpu_baseline(sites, budget_s, nk1, step_limit):
+    """The reference itself (oracle/_ref/cor, built from its sources) on a bounded
+    sample of the same sites: whole sites, -p (OpenMP over the rigidity scan) on this
+    host's cores, until the time budget is spent.  Returns the rate and, per sampled
+    site, the reference's (Rl, Ru, Rc)."""
+    ref = REPO / "oracle" / "_ref" / "cor"
+    if not ref.exists():
+        return None, {}
+    rng = np.random.default_rng(12345)
+    order = rng.permutation(len(sites))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    done, n_traj, t_total, cuts = 0, 0, 0.0, {}
+    with tempfile.TemporaryDirectory() as td:
+        for k in order:
+            r0, lat, lon = sites[k]
+            inp = Path(td) / "in.txt"
+            inp.write_text(f"0.01 -1 20.00\n{r0!r} {lat!r} {lon!r}\n{lat!r} {lon!r}\n"
+                           f"2020 1 1 1 0 0 0\n{nk1} 0 0 0.01\n-20 2 0 -2\n")
+            t0 = time.perf_counter()
+            res = subprocess.run([str(ref), "-p", "-i", "igrf14", "-e", "t96", "-s", str(step_limit), str(inp)],
+                                 check=True, capture_output=True, timeout=600, text=True)
+            t_total += time.perf_counter() - t0
+            lines = res.stdout.splitlines()
+            for j, ln in enumerate(lines):
+                if "CUTOFF with rigidities" in ln:
+                    cuts[int(k)] = tuple(float(v) for v in lines[j + 1].split())
+            done += 1
+            n_traj += 2000
+            if t_total >= budget_s and done >= 2:
+                break
+    return ({"value": n_traj / t_total, "unit": "trajectories/s", "cores": threads, "kind": "reference",
+             "sample": f"{done} seeded sites of the benchmark grid x 2000 rigidities "
+                       f"({n_traj} trajectories) in {t_total:.1f} s, oracle/_ref/cor -p"}, cuts)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

@@ -553,9 +553,10 @@ __device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool ac
 //     slot order) and waits until the slot is reserved and published.
 //   * A busy wavefront takes tickets for its free groups only for slots already
 //     reserved and not yet claimed (compare-and-swap on the ticket counter).
-//   * A wavefront running two or more trajectories while another waits hands one back
-//     (re-park, at most one park slot per lane of the grid in all), so the trajectories
-//     spread one per wavefront as the work thins out.
+//   * A wavefront running two or more trajectories while another waits hands back the
+//     one with the most steps (re-park, at most one park slot per lane of the grid in
+//     all), so the trajectories spread one per wavefront as the work thins out, the
+//     longest-running first.
 // An idle wavefront keeps its ticket until every trajectory of the run has finished
 // (a counter both modes add to), so a trajectory handed back is always taken; then it
 // retires.  Every slot is served by the holder of its ticket, so the count is reached;
@@ -671,7 +672,18 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
             const unsigned long long head1 = load_relaxed_uniform(&a.coop->head);
             const unsigned long long res1 = load_relaxed_uniform(&a.coop->res);
             if (head1 > res1) {
-                const int src = 63 - __clzll((long long)has_leads);    // lead lane of the last busy group
+                // the trajectory with the most steps so far (most likely the longest
+                // to go) moves to the waiting wavefront, where it runs alone
+                int src = -1;
+                unsigned best = 0;
+                for (uint64_t m = has_leads; m; m &= m - 1) {
+                    const int l = __ffsll((unsigned long long)m) - 1;
+                    const unsigned s_l = (unsigned)__shfl((int)step, l);
+                    if (src < 0 || s_l > best) {
+                        best = s_l;
+                        src = l;
+                    }
+                }
                 unsigned long long r = 0;
                 if (lane == src) r = atomicAdd(&a.coop->reparks, 1ull);
                 r = __shfl(r, src);
