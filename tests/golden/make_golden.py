@@ -8,6 +8,7 @@ reference sources (make -C oracle ref first):
     python tests/golden/make_golden.py          (everything)
     python tests/golden/make_golden.py ts05     (the TS05 fixtures only)
     python tests/golden/make_golden.py chaos    (the SH synthesis fixtures only)
+    python tests/golden/make_golden.py field    (the field-map run only)
 """
 import sys
 from pathlib import Path
@@ -106,7 +107,23 @@ def chaos_fixtures():
     print("chaos vectors", {k: v.shape for k, v in out.items()})
 
 
+def field_fixtures():
+    """One field-map run of the reference CLI (-m field, src/field.c:6-54): IGRF-14 +
+    T96 on a 20 x 30 degree grid at 1.5 Re, input and output text."""
+    import subprocess
+    name = "field_igrf_t96"
+    inp = HERE / f"{name}.in"
+    inp.write_text("1.5 -80.0 0.0\n20.0 30.0 80.0 360.0\n2020 1 1 1 0 0 0\n-20 2 0 -2\n")
+    txt = subprocess.run([str(L.REF_BIN), "-m", "field", "-i", "igrf14", "-e", "t96", str(inp)],
+                         capture_output=True, text=True, timeout=120, check=True).stdout
+    (HERE / f"{name}.out").write_text(txt)
+    print(name, len(txt.splitlines()), "lines")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "field":
+        field_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "ts05":
         ts05_fixtures()
         return
@@ -154,6 +171,7 @@ def main():
         print(name, txt.strip().splitlines()[-1])
     ts05_fixtures()
     chaos_fixtures()
+    field_fixtures()
 
 
 if __name__ == "__main__":

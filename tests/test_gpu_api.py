@@ -1,0 +1,196 @@
+"""The reference's own C API on the GPU, driven as the reference's command line drives
+it (src/main.c:12-93 for cutoffs, :95-141 for field maps): open the input file, read
+the header, find the models in the registries (get_model), let each model's setup_fun
+read its lines, write the output header, then trajectory_simulation_cutoff or
+field_simulation into a FILE -- and the text is compared with the reference's own
+output of the same input (tests/golden/*.out, written by the reference built from its
+sources).
+
+Bar, cutoffs: the header is identical; outside the union penumbra band the allowed
+rigidities are identical; from 1 GV above the band every printed column agrees (v/c to
+1e-9, exit radius to 1e-4 Re, angles to 0.01 degree, flight time and path length to
+1e-5 relative); the summary's Rl, Ru, Rc within the test's rigidity tolerance.  Field
+map: header identical, every value within 0.01 nT of the printed reference value.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import cor_amd as A
+import oracle_lib as L
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+libc = C.CDLL(None)
+libc.fopen.restype = C.c_void_p
+libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+libc.fclose.argtypes = [C.c_void_p]
+
+SETUP = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
+class InfileHeader(C.Structure):
+    _fields_ = [("r0", C.c_double), ("the0", C.c_double), ("fi0", C.c_double), ("the1", C.c_double),
+                ("fi1", C.c_double), ("iy", C.c_int), ("mes", C.c_int), ("ide", C.c_int), ("id", C.c_int),
+                ("ih", C.c_int), ("min", C.c_int), ("is_", C.c_int), ("zn", C.c_double), ("nk1", C.c_int),
+                ("iopt", C.c_int), ("ist", C.c_int), ("rig", C.c_double), ("del_", C.c_double),
+                ("rk", C.c_double)]
+
+
+class Model(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("version", C.c_char_p), ("setup_data", C.c_void_p),
+                ("setup_fun", SETUP), ("calc_fun", C.c_void_p), ("kind", C.c_int), ("sh_nm_max", C.c_int32),
+                ("sh_g", C.c_void_p), ("sh_h", C.c_void_p)]
+
+
+def lib():
+    L_ = A.lib()
+    L_.get_model.restype = C.c_void_p
+    L_.get_model.argtypes = [C.c_void_p, C.c_char_p]
+    L_.trajectory_simulation_cutoff.restype = None
+    L_.trajectory_simulation_cutoff.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                                C.c_double, C.c_double, C.c_uint32]
+    L_.field_simulation.restype = None
+    L_.field_simulation.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+    return L_
+
+
+def registry_models(L_, internal, external):
+    """NULL-terminated model list as src/main.c builds it from -i / -e."""
+    models = (C.c_void_p * 3)()
+    k = 0
+    for reg, ver in (("internal_models", internal), ("external_models", external)):
+        if ver:
+            m = L_.get_model(C.addressof(C.c_void_p.in_dll(L_, reg)), ver.encode())
+            assert m, ver
+            models[k] = m
+            k += 1
+    return models
+
+
+def setup_models(models, f, header, params):
+    """src/main.c:51-62: every model's setup_fun reads its lines of the input file."""
+    for m in models:
+        if not m:
+            break
+        md = Model.from_address(m)
+        assert md.setup_fun(f, C.addressof(header), C.addressof(params), md.setup_data) == 0
+
+
+def run_cutoff(L_, tmp_path, name, internal, external, steps):
+    inp = GOLD / f"{name}.in"
+    h = InfileHeader()
+    p = A.ModelsParams()
+    models = registry_models(L_, internal, external)
+    f = libc.fopen(str(inp).encode(), b"r")
+    assert f
+    try:
+        assert L_.read_cutoff_infile_header(C.c_void_p(f), C.byref(h)) == 0
+        setup_models(models, f, h, p)
+    finally:
+        libc.fclose(f)
+    out = tmp_path / f"{name}.out"
+    g = libc.fopen(str(out).encode(), b"w")
+    try:
+        assert L_.write_cutoff_outfile_header(C.c_void_p(g), C.byref(h), models, C.c_uint64(steps)) == 0
+        L_.trajectory_simulation_cutoff(C.byref(h), C.cast(models, C.c_void_p), C.byref(p), g, steps, 0.01,
+                                        A.M_PROTON, 1)
+    finally:
+        libc.fclose(g)
+    return out.read_text(), h
+
+
+def header_part(text):
+    lines = text.splitlines()
+    for k, ln in enumerate(lines):
+        if len(ln.split()) == 9:
+            try:
+                [float(v) for v in ln.split()]
+                return lines[:k]
+            except ValueError:
+                pass
+    return lines
+
+
+CUTOFF_RUNS = [
+    ("config1_igrf_nk100", "igrf14", None, 1000000, 0.01),
+    ("t96_equator_nk400", "igrf14", "t96", 1000000, 0.05),
+    ("t96_storm_limit_nk100", "igrf14", "t96", 20000, 0.25),
+    ("ts05_storm_nk100", "igrf14", "ts05", 20000, 0.25),
+]
+
+
+@pytest.mark.parametrize("name,internal,external,steps,rc_tol", CUTOFF_RUNS, ids=[r[0] for r in CUTOFF_RUNS])
+def test_trajectory_simulation_cutoff_text(tmp_path, name, internal, external, steps, rc_tol):
+    A.init(0)
+    L_ = lib()
+    mine, h = run_cutoff(L_, tmp_path, name, internal, external, steps)
+    ref = (GOLD / f"{name}.out").read_text()
+    assert header_part(mine) == header_part(ref)
+    got, cut_g = L.parse_cutoff_output(mine)
+    want, cut_r = L.parse_cutoff_output(ref)
+    print(name, "gpu", cut_g, "reference", cut_r)
+    assert all(abs(a - b) <= rc_tol + 1e-9 for a, b in zip(cut_g, cut_r)), (cut_g, cut_r)
+    lo, hi = min(cut_g[0], cut_r[0]), max(cut_g[1], cut_r[1])
+    key = lambda r: int(round((r - h.rig) / h.del_))        # noqa: E731
+    g_out = {key(r["rig"]) for r in got if r["rig"] < lo - 1e-9 or r["rig"] > hi + 1e-9}
+    r_out = {key(r["rig"]) for r in want if r["rig"] < lo - 1e-9 or r["rig"] > hi + 1e-9}
+    assert g_out == r_out, sorted(g_out ^ r_out)[:10]
+    gi = {key(r["rig"]): r for r in got}
+    n = 0
+    for r in want:
+        if r["rig"] <= hi + 1.0:
+            continue
+        g = gi[key(r["rig"])]
+        assert abs(g["vc"] - r["vc"]) <= 1e-9
+        assert abs(g["r"] - r["r"]) <= 1e-4
+        for c in ("td", "fei", "ast", "asf"):
+            d = abs(g[c] - r[c])
+            assert min(d, 360.0 - d) <= 0.01, (r["rig"], c, g[c], r[c])
+        for c in ("time", "length_km"):
+            assert abs(g[c] - r[c]) <= 1e-5 * abs(r[c]) + 0.011, (r["rig"], c, g[c], r[c])
+        n += 1
+    print(name, n, "lines compared column by column")
+
+
+def test_field_simulation_text(tmp_path):
+    A.init(0)
+    L_ = lib()
+    name = "field_igrf_t96"
+    inp = GOLD / f"{name}.in"
+    h = InfileHeader()
+    p = A.ModelsParams()
+    models = registry_models(L_, "igrf14", "t96")
+    f = libc.fopen(str(inp).encode(), b"r")
+    try:
+        assert L_.read_field_infile_header(C.c_void_p(f), C.byref(h)) == 0
+        setup_models(models, f, h, p)
+    finally:
+        libc.fclose(f)
+    out = tmp_path / "field.out"
+    g = libc.fopen(str(out).encode(), b"w")
+    try:
+        assert L_.write_field_outfile_header(C.c_void_p(g), C.byref(h), models) == 0
+        L_.field_simulation(C.byref(h), C.byref(p), C.cast(models, C.c_void_p), g, 1)
+    finally:
+        libc.fclose(g)
+    mine = out.read_text().splitlines()
+    ref = (GOLD / f"{name}.out").read_text().splitlines()
+    assert len(mine) == len(ref)
+    nums = 0
+    for a, b in zip(mine, ref):
+        fa, fb = a.split(), b.split()
+        if not fa or not fb:
+            assert a.strip() == b.strip()
+            continue
+        try:
+            va, vb = np.array(fa, dtype=float), np.array(fb, dtype=float)
+        except ValueError:
+            assert a == b
+            continue
+        assert va.shape == vb.shape and np.all(np.abs(va - vb) <= 0.0100001), (a, b)
+        nums += 1
+    assert nums == 9 * 12
