@@ -37,7 +37,8 @@ EXPORTED = [
     "cor_trace_plan_run", "cor_trace_plan_sync", "cor_trace_plan_count", "cor_trace_plan_results",
     "cor_trace_plan_stats", "cor_trace_plan_destroy", "cor_gpu_trace", "cor_gpu_field",
     "cor_scan_count", "cor_geodetic_to_geocentric", "cor_direction_to_geo", "cor_gpu_synchronize",
-    "cor_trace_plan_set_epoch", "cor_gpu_trace_series", "cor_refine_scan",
+    "cor_trace_plan_set_epoch", "cor_gpu_trace_series", "cor_refine_scan", "cor_gpu_trace_tracks",
+    "trajectory_simulation",
 ]
 
 
@@ -88,6 +89,8 @@ TRAJ_DTYPE = np.dtype([("status", "<i4"), ("steps", "<u4"), ("attempts", "<u4"),
 SCAN_DTYPE = np.dtype([("r0", "<f8"), ("the0", "<f8"), ("fi0", "<f8"), ("the1", "<f8"),
                        ("fi1", "<f8"), ("rig", "<f8"), ("del_", "<f8"), ("rk", "<f8"),
                        ("zn", "<f8"), ("nk1", "<i4"), ("pad", "<i4")])
+TRACK_DTYPE = np.dtype([("step", "<u4"), ("pad", "<u4"), ("gsm", "<f8", (3,)), ("r", "<f8"), ("td", "<f8"),
+                        ("fei", "<f8"), ("v", "<f8"), ("time", "<f8"), ("length_m", "<f8")])
 CUT_DTYPE = np.dtype([("rl", "<f8"), ("ru", "<f8"), ("rc", "<f8"), ("n_forbidden", "<i4"),
                       ("n", "<i4")])
 
@@ -130,6 +133,9 @@ def lib():
     L.cor_trace_plan_set_epoch.argtypes = [C.c_void_p, C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p]
     L.cor_gpu_trace_series.argtypes = [C.POINTER(ModelSet), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                        C.POINTER(TraceOpts), C.c_int32, C.c_void_p, C.c_void_p]
+    L.cor_gpu_trace_tracks.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p, C.c_int32,
+                                       C.POINTER(TraceOpts), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p,
+                                       C.c_int64]
     L.cor_gpu_field.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), DP, DP, C.c_int64, C.c_int32]
     L.cor_geodetic_to_geocentric.argtypes = [C.c_double, C.c_double, DP, DP]
     L.cor_direction_to_geo.argtypes = [C.c_double] * 4 + [DP, DP]
@@ -272,6 +278,27 @@ def trace_series(models, epochs, scans, in_flight=0, step_limit=1000000, tu_angl
                                      len(scans), C.byref(opts), in_flight, tr.ctypes.data if traj else None,
                                      cut.ctypes.data), "cor_gpu_trace_series")
     return tr, cut
+
+
+def trace_tracks(models, params, scans, step_limit=1000000, tu_angle=0.01, mass=M_PROTON):
+    """cor_gpu_trace_tracks: the per-step tracks of every trajectory (FP64), as a list
+    of TRACK_DTYPE arrays in scan order."""
+    scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
+    n = sum(scan_count(r) for r in scans)
+    opts = TraceOpts(step_limit, tu_angle, mass, PREC_FP64, 0, 0, 0)
+    counts = np.zeros(max(n, 1), dtype=np.int64)
+    total = C.c_int64()
+    check(lib().cor_gpu_trace_tracks(C.byref(models), C.byref(params), scans.ctypes.data, len(scans), C.byref(opts),
+                                     counts.ctypes.data, C.byref(total), None, 0), "cor_gpu_trace_tracks")
+    pts = np.zeros(max(total.value, 1), dtype=TRACK_DTYPE)
+    check(lib().cor_gpu_trace_tracks(C.byref(models), C.byref(params), scans.ctypes.data, len(scans), C.byref(opts),
+                                     counts.ctypes.data, C.byref(total), pts.ctypes.data, len(pts)),
+          "cor_gpu_trace_tracks")
+    out, k = [], 0
+    for c in counts[:n]:
+        out.append(pts[k:k + c])
+        k += c
+    return out
 
 
 def field(models, params, gsm_m, precision=PREC_FP64):

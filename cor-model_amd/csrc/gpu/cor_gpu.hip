@@ -275,6 +275,8 @@ struct cor_trace_plan {
     RunStats *d_run;
     hipEvent_t ev[3];
     bool ran;
+    cor_track_point *d_trk;      // track recording of the next runs (NULL: off)
+    int64_t *d_trk_off;
     EpochDev *h_epoch;           // pinned staging of set_epoch uploads
     hipEvent_t ev_up;            // the last set_epoch upload has left h_epoch
     bool up_pending;
@@ -351,6 +353,8 @@ void cor_trace_plan_destroy(cor_trace_plan *p) {
     (void)hipFree(p->d_coop);
     (void)hipFree(p->d_park);
     (void)hipFree(p->d_run);
+    (void)hipFree(p->d_trk);
+    (void)hipFree(p->d_trk_off);
     for (int k = 0; k < 3; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     if (p->up_pending) (void)hipEventSynchronize(p->ev_up);
@@ -518,6 +522,8 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.park = p->d_park;
     a.stats = p->d_run;
     a.repark_limit = (long long)p->grid * p->block;
+    a.trk = p->d_trk;
+    a.trk_off = p->d_trk_off;
     a.idle_weight = COOP_IDLE_WEIGHT;
     reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop, p->d_run);
     HIP_TRY(hipGetLastError());
@@ -695,6 +701,62 @@ int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const 
     if (rc) return rc;
     rc = cor_trace_plan_run(p, nullptr);
     if (!rc) rc = cor_trace_plan_results(p, traj, cut);
+    cor_trace_plan_destroy(p);
+    return rc;
+}
+
+int cor_gpu_trace_tracks(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                         int32_t n_scans, const cor_trace_opts *opts, int64_t *counts, int64_t *total,
+                         cor_track_point *points, int64_t capacity) {
+    if (!opts || !total || opts->precision != COR_PREC_FP64 || (points && capacity < 0)) return COR_E_INVALID;
+    cor_trace_plan *p = nullptr;
+    int rc = cor_trace_plan_create(models, epoch, scans, n_scans, opts, &p);
+    if (rc) return rc;
+    const int64_t n = p->n_items;
+    std::vector<cor_traj> tr((size_t)std::max<int64_t>(n, 1));
+    // pass 1: the trace itself; a trajectory ending at step S has S - 1 points
+    rc = cor_trace_plan_run(p, nullptr);
+    if (!rc) rc = cor_trace_plan_results(p, tr.data(), nullptr);
+    std::vector<int64_t> off((size_t)n + 1, 0);
+    if (!rc) {
+        for (int64_t i = 0; i < n; i++) off[i + 1] = off[i] + (tr[i].steps > 0 ? (int64_t)tr[i].steps - 1 : 0);
+        *total = off[n];
+        if (counts)
+            for (int64_t i = 0; i < n; i++) counts[i] = off[i + 1] - off[i];
+        if (points && capacity < off[n]) rc = COR_E_NOMEM;
+    }
+    if (!rc && points && off[n] > 0) {
+        // pass 2: the same trace (FP64 is deterministic) recording every point
+        const size_t bytes = sizeof(cor_track_point) * (size_t)off[n];
+        hipError_t e = hipMalloc(&p->d_trk, bytes);
+        if (e == hipSuccess) e = hipMalloc(&p->d_trk_off, sizeof(int64_t) * ((size_t)n + 1));
+        if (e == hipSuccess) e = hipMemset(p->d_trk, 0, bytes);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_trk_off, off.data(), sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            rc = fail_hip(e, "cor_gpu_trace_tracks");
+        } else {
+            rc = cor_trace_plan_run(p, nullptr);
+            if (!rc) rc = cor_trace_plan_sync(p);
+            if (!rc) {
+                e = hipMemcpy(points, p->d_trk, bytes, hipMemcpyDeviceToHost);
+                if (e != hipSuccess) rc = fail_hip(e, "cor_gpu_trace_tracks");
+            }
+        }
+        if (!rc) {
+            // a trajectory whose state turned NaN wrote its points up to there only:
+            // the written points are the leading ones with a step number
+            int64_t t = 0;
+            for (int64_t i = 0; i < n; i++) {
+                int64_t m = 0;
+                while (m < off[i + 1] - off[i] && points[off[i] + m].step != 0u) m++;
+                if (counts) counts[i] = m;
+                for (int64_t j = 0; j < m && t != off[i]; j++) points[t + j] = points[off[i] + j];
+                t += m;
+            }
+            *total = t;
+        }
+    }
     cor_trace_plan_destroy(p);
     return rc;
 }

@@ -12,6 +12,7 @@
 //              written once per trajectory).
 #pragma once
 #include <stdint.h>
+#include "cor_gpu.h"
 #include "t96_dev.hpp"
 #include "ts05_dev.hpp"
 
@@ -153,6 +154,8 @@ struct TraceArgs {
     ParkDev *park;
     RunStats *stats;
     long long repark_limit;      // park slots beyond the per-lane ones (one per lane of the grid)
+    cor_track_point *trk;        // FP64 track recording (NULL: off)
+    const int64_t *trk_off;      // first point of each trajectory (n_items + 1 entries)
     long long idle_weight;       // trajectories one waiting worker is counted to take
 };
 

@@ -320,6 +320,39 @@ __device__ __forceinline__ void write_end(const TraceArgs &a, int64_t gid, int s
     }
 }
 
+// One line of the trajectory mode (src/trajectory.c:232-269): the GSM position of the
+// new point (geo_to_gsm, src/utils.c:75-87), its spherical coordinates (car_to_sph,
+// src/utils.c:43-73), speed, time and path length after an accepted step that does not
+// end the trajectory.
+__device__ __forceinline__ void write_track(const TraceArgs &a, int64_t gid, unsigned int step, const double *A,
+                                            v3<double> pos, double v, double time, double alength) {
+    const int64_t k = a.trk_off[gid] + (int64_t)step - 1;
+    if (step < 1u || k >= a.trk_off[gid + 1]) return;
+    const double x = pos.x, y = pos.y, z = pos.z;
+    const double sq = x * x + y * y;
+    const double rr = sqrt(sq + z * z);
+    double th = 0., ph = 0.;
+    if (sq != 0.) {
+        ph = atan2(y, x);
+        th = atan2(sqrt(sq), z);
+        if (ph < 0.) ph += K_2PI;
+    } else if (z < 0.) {
+        th = K_PI;
+    }
+    cor_track_point &t = a.trk[k];
+    t.step = step;
+    t.pad = 0u;
+    t.gsm[0] = A[0] * x + A[1] * y + A[2] * z;
+    t.gsm[1] = A[3] * x + A[4] * y + A[5] * z;
+    t.gsm[2] = A[6] * x + A[7] * y + A[8] * z;
+    t.r = rr / K_RE;
+    t.td = 90. - (th / K_TO_RAD);
+    t.fei = ph / K_TO_RAD;
+    t.v = v;
+    t.time = time;
+    t.length_m = alength;
+}
+
 __device__ __forceinline__ void write_exit(const TraceArgs &a, int64_t gid, v3<double> pos, v3<double> vel,
                                            double v, double r, double time, double alength) {
     const double rho2 = pos.x * pos.x + pos.y * pos.y;
@@ -894,6 +927,8 @@ trace_kernel(TraceArgs a) {
                 write_end(a, gid, status, step, attempts);
                 active = false;
                 ended_now = true;
+            } else if constexpr (PREC == PREC_FP64) {
+                if (a.trk) write_track(a, gid, step, E.A, pos, v, time, alength);
             }
         }
         // trajectories ended here, added to the run's count when the wavefront leaves

@@ -39,3 +39,45 @@ void trajectory_simulation_cutoff(InfileHeader *header, const Model **models, Mo
     fprintf(outfile, "  CUTOFF with rigidities P(S),P(C),P(M) are:\n%12.5f%12.5f%12.5f\n\n", cut.rl, cut.ru, cut.rc);
     free(traj);
 }
+
+/* src/trajectory.c:163-273 (the trajectory mode): the tracks of the header's scan,
+ * traced on the GPU in FP64, one line per accepted step that does not end the
+ * trajectory, to tragsm (GSM position [m]) and trasph (radius [Re], latitude,
+ * longitude [deg]), each followed by speed, flight time and path length, in the
+ * reference's formats. */
+void trajectory_simulation(InfileHeader *header, const Model **models, ModelsParams *models_params, FILE *tragsm,
+                           FILE *trasph, uint64_t step_limit, double tu_angle, double particle_mass) {
+    cor_model_set set;
+    const cor_scan scan = {header->r0, header->the0, header->fi0, header->the1, header->fi1,
+                           header->rig, header->del, header->rk, header->zn, header->nk1, 0};
+    const cor_trace_opts opts = {step_limit, tu_angle, particle_mass, COR_PREC_FP64, 0, 0, 0};
+    int rc = cor_models_to_set(models, &set);
+    const int64_t n = cor_scan_count(&scan);
+    if (rc == COR_OK && n < 0) rc = COR_E_INVALID;
+    int64_t total = 0;
+    int64_t *counts = NULL;
+    cor_track_point *pts = NULL;
+    if (rc == COR_OK) {
+        counts = (int64_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(int64_t));
+        rc = counts ? cor_gpu_trace_tracks(&set, models_params, &scan, 1, &opts, counts, &total, NULL, 0)
+                    : COR_E_NOMEM;
+    }
+    if (rc == COR_OK && total > 0) {
+        pts = (cor_track_point *)malloc(sizeof(cor_track_point) * (size_t)total);
+        rc = pts ? cor_gpu_trace_tracks(&set, models_params, &scan, 1, &opts, counts, &total, pts, total)
+                 : COR_E_NOMEM;
+    }
+    if (rc != COR_OK) {
+        fprintf(stderr, "Error: GPU tracer failed: %s (%s)\n", cor_gpu_error_string(rc), cor_gpu_last_hip_error());
+    } else {
+        for (int64_t k = 0; k < total; k++) {
+            const cor_track_point *t = &pts[k];
+            fprintf(tragsm, "%6d%17.5f%17.5f%17.5f%14.2f%15.10f%18.2f\n", (int)t->step, t->gsm[0], t->gsm[1],
+                    t->gsm[2], t->v, t->time, t->length_m);
+            fprintf(trasph, "%6d%17.5f%17.5f%17.5f%14.2f%15.10f%18.2f\n", (int)t->step, t->r, t->td, t->fei, t->v,
+                    t->time, t->length_m);
+        }
+    }
+    free(pts);
+    free(counts);
+}

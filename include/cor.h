@@ -160,6 +160,14 @@ void trajectory_simulation_cutoff(InfileHeader *header, const Model **models,
                                   uint64_t step_limit, double tu_angle, double particle_mass,
                                   uint32_t parallel);
 
+/* src/trajectory.h:11.  Every trajectory of the header's scan (the reference's
+ * trajectory mode has rig == rk: one) is traced on the GPU in FP64 and its track is
+ * written to tragsm (GSM position) and trasph (radius, latitude, longitude) line by line
+ * as the reference writes them.  GPU errors are reported on stderr. */
+void trajectory_simulation(InfileHeader *header, const Model **models, ModelsParams *models_params,
+                           FILE *tragsm, FILE *trasph, uint64_t step_limit, double tu_angle,
+                           double particle_mass);
+
 /* src/field.h:7 */
 void field_simulation(InfileHeader *header, ModelsParams *params, const Model **models,
                       FILE *outfile, uint32_t parallel);

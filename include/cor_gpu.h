@@ -95,6 +95,18 @@ typedef struct {
     double vc, r, td, fei, ast, asf, time, length_m;
 } cor_traj;
 
+/* One step of a trajectory's track: the values of one line of the reference's
+ * trajectory mode files (src/trajectory.c:265-269, tragsm_* and trasph_*). */
+typedef struct {
+    uint32_t step;            /* accepted step number, from 1 */
+    uint32_t pad;
+    double gsm[3];            /* GSM position [m] */
+    double r, td, fei;        /* radius [Re], GEO latitude and longitude [deg] */
+    double v;                 /* speed [m/s] */
+    double time;              /* flight time [s] */
+    double length_m;          /* path length [m] */
+} cor_track_point;
+
 /* Per scan: the reference's P(S), P(C), P(M) = Rl, Ru, Rc (src/trajectory.c:156-160). */
 typedef struct {
     double rl, ru, rc;
@@ -164,6 +176,18 @@ int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const 
 int cor_gpu_trace_series(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
                          const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
                          int32_t in_flight, cor_traj *traj, cor_cutoff *cut);
+
+/* Per-step tracks of every trajectory of the scans in FP64, as the reference's
+ * trajectory mode writes them (trajectory_simulation, src/trajectory.c:163-273): one
+ * point per accepted step that does not end the trajectory.  counts (nullable) receives
+ * the number of points of each trajectory (cor_scan_count entries per scan, scan
+ * order) and *total their sum; points receives them, trajectory after trajectory.  With
+ * points == NULL only the counts (upper bounds: a trajectory whose state turns NaN stops
+ * there, where the reference would print NaN lines up to the step limit) are returned;
+ * COR_E_NOMEM when capacity < *total.  opts->precision must be COR_PREC_FP64. */
+int cor_gpu_trace_tracks(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                         int32_t n_scans, const cor_trace_opts *opts, int64_t *counts, int64_t *total,
+                         cor_track_point *points, int64_t capacity);
 
 /* calculate_field over n points: GSM position [m] (n x 3) -> GEO field [nT] (n x 3). */
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,

@@ -9,6 +9,7 @@ reference sources (make -C oracle ref first):
     python tests/golden/make_golden.py ts05     (the TS05 fixtures only)
     python tests/golden/make_golden.py chaos    (the SH synthesis fixtures only)
     python tests/golden/make_golden.py field    (the field-map run only)
+    python tests/golden/make_golden.py track    (the trajectory-mode runs only)
 """
 import sys
 from pathlib import Path
@@ -120,7 +121,37 @@ def field_fixtures():
     print(name, len(txt.splitlines()), "lines")
 
 
+TRACK_RUNS = {
+    # name: (site r0, lat, lon), rigidity [GV], nk1, external model and its lines, step limit
+    "track_t96_5gv": ((1.00260, 49.8074, 120.0), 5.0, 400, "t96", [-20, 2, 0, -2], 20000),
+    "track_t96_trapped": ((1.00260, 49.8074, 120.0), 0.3, 100, "t96", [-20, 2, 0, -2], 3000),
+    "track_ts05_3gv": ((1.00280, 52.0, 30.0), 3.0, 100, "ts05", [-100, 3, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5],
+                       20000),
+}
+
+
+def track_fixtures():
+    """Trajectory-mode runs of the reference CLI (-m trajectory, src/trajectory.c:163-273):
+    one rigidity each, the tragsm_* and trasph_* files it writes."""
+    import subprocess
+    import tempfile
+    for name, (site, rig, nk1, ext, params, steps) in TRACK_RUNS.items():
+        r0, lat, lon = site
+        inp = HERE / f"{name}.in"
+        L.write_cutoff_input(inp, rig, -1.0, rig, r0, lat, lon, lat, lon, nk1=nk1, dl=0.01, ext_params=params)
+        with tempfile.TemporaryDirectory() as td:
+            subprocess.run([str(L.REF_BIN), "-m", "trajectory", "-i", "igrf14", "-e", ext, "-s", str(steps),
+                            "-o", name, str(inp)], cwd=td, check=True, timeout=300, capture_output=True)
+            for kind in ("tragsm", "trasph"):
+                txt = (Path(td) / f"{kind}_{name}").read_text()
+                (HERE / f"{name}.{kind}").write_text(txt)
+        print(name, len(txt.splitlines()), "lines")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "track":
+        track_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "field":
         field_fixtures()
         return
@@ -172,6 +203,7 @@ def main():
     ts05_fixtures()
     chaos_fixtures()
     field_fixtures()
+    track_fixtures()
 
 
 if __name__ == "__main__":

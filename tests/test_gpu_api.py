@@ -194,3 +194,53 @@ def test_field_simulation_text(tmp_path):
         assert va.shape == vb.shape and np.all(np.abs(va - vb) <= 0.0100001), (a, b)
         nums += 1
     assert nums == 9 * 12
+
+
+TRACK_RUNS = [("track_t96_5gv", "t96", 20000), ("track_t96_trapped", "t96", 3000), ("track_ts05_3gv", "ts05", 20000)]
+
+
+@pytest.mark.parametrize("name,external,steps", TRACK_RUNS, ids=[r[0] for r in TRACK_RUNS])
+def test_trajectory_simulation_tracks(tmp_path, name, external, steps):
+    """The trajectory mode (src/main.c:143-232, src/trajectory.c:163-273): one
+    trajectory's track in GSM and spherical coordinates, line for line against the
+    reference's files.  Same number of lines and step numbers; the values within 1e-9
+    of their scale over the first half of the track and 1e-6 over all of it (last-bit
+    differences of the transcendentals grow along a path; the trapped track runs to the
+    step limit)."""
+    A.init(0)
+    L_ = lib()
+    L_.trajectory_simulation.restype = None
+    L_.trajectory_simulation.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                         C.c_double, C.c_double]
+    inp = GOLD / f"{name}.in"
+    h = InfileHeader()
+    p = A.ModelsParams()
+    models = registry_models(L_, "igrf14", external)
+    f = libc.fopen(str(inp).encode(), b"r")
+    try:
+        assert L_.read_cutoff_infile_header(C.c_void_p(f), C.byref(h)) == 0
+        setup_models(models, f, h, p)
+    finally:
+        libc.fclose(f)
+    outs = {k: tmp_path / f"{k}_{name}" for k in ("tragsm", "trasph")}
+    g1 = libc.fopen(str(outs["tragsm"]).encode(), b"w")
+    g2 = libc.fopen(str(outs["trasph"]).encode(), b"w")
+    try:
+        L_.trajectory_simulation(C.byref(h), C.cast(models, C.c_void_p), C.byref(p), g1, g2, steps, 0.01, A.M_PROTON)
+    finally:
+        libc.fclose(g1)
+        libc.fclose(g2)
+    for kind in ("tragsm", "trasph"):
+        mine = np.loadtxt(outs[kind], ndmin=2)
+        ref = np.loadtxt(GOLD / f"{name}.{kind}", ndmin=2)
+        assert mine.shape == ref.shape, (kind, mine.shape, ref.shape)
+        assert np.array_equal(mine[:, 0], ref[:, 0])
+        # one unit in the last printed digit of each column ("%17.5f" x3, "%14.2f",
+        # "%15.10f", "%18.2f") is always allowed
+        unit = np.array([1e-5, 1e-5, 1e-5, 1e-2, 1e-10, 1e-2])
+        scale = np.maximum(np.abs(ref[:, 1:]).max(axis=0), 1e-30)
+        excess = np.maximum(np.abs(mine[:, 1:] - ref[:, 1:]) - 1.0001 * unit, 0.0) / scale
+        half = len(ref) // 2
+        print(name, kind, len(ref), "lines; max deviation beyond the printed unit / scale: first half",
+              excess[:half].max(), "all", excess.max())
+        assert excess[:half].max() <= 1e-9 and excess.max() <= 1e-6
