@@ -156,6 +156,9 @@ def main():
     ap.add_argument("--queue-order", type=int, default=1)
     ap.add_argument("--tail-lanes", type=int, default=0,
                     help="cooperative tail threshold (0 = library default, -1 = per-lane mode only)")
+    ap.add_argument("--handover-steps", type=int, default=0,
+                    help="per-lane steps after which a trajectory is handed to the cooperative mode "
+                         "on its own once the queue is drained (0: library default, negative: off)")
     ap.add_argument("--fp64-stride", type=int, default=16,
                     help="FP64 comparison run on every fp64-stride-th site of the benchmarked sites")
     ap.add_argument("--no-fp64", action="store_true")
@@ -180,7 +183,8 @@ def main():
     params = A.recalc(*EPOCH, parmod=PARMOD)
     models = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
     mixed = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_MIXED,
-                   queue_order=args.queue_order, tail_lanes=args.tail_lanes)
+                   queue_order=args.queue_order, tail_lanes=args.tail_lanes,
+                   handover_steps=args.handover_steps)
     n = mixed.n
     print(f"[bench] {len(scans)} sites, {n} trajectories; warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
@@ -215,7 +219,8 @@ def main():
         # the mixed path on the same sites, for a like-for-like speed ratio (both runs
         # of a site subset end with the same slowest trajectories)
         msub = A.Plan(models, params, scans[sub], step_limit=args.step_limit, precision=A.PREC_MIXED,
-                      queue_order=args.queue_order, tail_lanes=args.tail_lanes)
+                      queue_order=args.queue_order, tail_lanes=args.tail_lanes,
+                      handover_steps=args.handover_steps)
         t_msub, _ = time_plan(msub, 1, 0)
         msub.close()
         drc = np.abs(cut_m["rc"][sub] - cut_d["rc"])
@@ -272,7 +277,8 @@ def main():
                                "(Pdyn 2, Dst -20, By 0, Bz -2), 0.01-20 GV step 0.01",
                    "sites": len(scans), "site_stride": args.stride, "trajectories": int(n),
                    "nk1": args.nk1, "step_limit": args.step_limit, "tu_angle": 0.01,
-                   "queue_order": args.queue_order, "tail_lanes": args.tail_lanes},
+                   "queue_order": args.queue_order, "tail_lanes": args.tail_lanes,
+                   "handover_steps": args.handover_steps},
         "kernel_ms": k_mixed,
         "integration_steps_per_s": st_m.steps / (k_mixed * 1e-3),
         # where the tracer's time goes: mean live lanes per per-lane wavefront step,

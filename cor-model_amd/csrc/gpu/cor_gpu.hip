@@ -119,17 +119,42 @@ void build_epoch(const cor_model_set *m, const ModelsParams *p, EpochDev &E) {
         t96_params<double>(p->PARMOD, E.t96d);
         t96_params<float>(p->PARMOD, E.t96f);
     }
-    if (m->crustal == COR_CRUSTAL_SH) {
-        const int nm = m->sh_nm_max;
-        E.sh_nm = nm;
-        for (int n = 0; n < nm; n++)
-            for (int k = 0; k < nm; k++) {
-                E.shG[n * SH_NM_MAX + k] = m->sh_g[n * nm + k];
-                E.shH[n * SH_NM_MAX + k] = m->sh_h[n * nm + k];
-                E.shGf[n * SH_NM_MAX + k] = (float)m->sh_g[n * nm + k];
-                E.shHf[n * SH_NM_MAX + k] = (float)m->sh_h[n * nm + k];
-            }
+}
+
+// The SH tables of a model set in device memory, [nm][nm] as the caller gives them:
+// FP64 G and H, then FP32 G and H, in one block (reused when large enough), pointed to
+// by the epoch.  Synchronous: no kernel may be reading the block.
+hipError_t upload_sh(const cor_model_set *m, void **buf, size_t *cap, EpochDev &E) {
+    if (m->crustal != COR_CRUSTAL_SH) return hipSuccess;
+    const int nm = m->sh_nm_max;
+    const size_t nn = (size_t)nm * (size_t)nm;
+    const size_t bytes = nn * (2 * sizeof(double) + 2 * sizeof(float));
+    if (*cap < bytes) {
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+        const hipError_t e = hipMalloc(buf, bytes);
+        if (e != hipSuccess) return e;
+        *cap = bytes;
     }
+    std::vector<double> d(2 * nn);
+    std::vector<float> f(2 * nn);
+    for (size_t k = 0; k < nn; k++) {
+        d[k] = m->sh_g[k];
+        d[nn + k] = m->sh_h[k];
+        f[k] = (float)m->sh_g[k];
+        f[nn + k] = (float)m->sh_h[k];
+    }
+    double *dd = (double *)*buf;
+    float *ff = (float *)(dd + 2 * nn);
+    hipError_t e = hipMemcpy(dd, d.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ff, f.data(), sizeof(float) * 2 * nn, hipMemcpyHostToDevice);
+    E.sh_nm = nm;
+    E.shG = dd;
+    E.shH = dd + nn;
+    E.shGf = ff;
+    E.shHf = ff + nn;
+    return e;
 }
 
 // Station position and arrival-direction trig as src/trajectory.c:18-31 and
@@ -241,6 +266,9 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
 // 4 and 1, 17.5 s with 32 and 64).
 constexpr int COOP_LANES_DEFAULT = 32;
 constexpr long long COOP_IDLE_WEIGHT = 64;
+// After the drain, a trajectory that has run this many per-lane steps is handed over on
+// its own (0: off).
+constexpr uint32_t COOP_LONG_STEPS_DEFAULT = 0u;
 
 // the mixed T96 evaluator reads its interconnection series from the lane tables
 bool needs_coop_tables(int precision, int ext) { return precision != COR_PREC_FP64 && ext == EXT_T96; }
@@ -269,12 +297,15 @@ struct cor_trace_plan {
     double tu_angle, cos_tu, mass;
     int grid, block;
     int coop_lanes;              // 0: per-lane mode only
+    uint32_t long_steps;         // per-trajectory handover after this many steps (0: off)
     uint32_t run_id;
     CoopCtl *d_coop;
     ParkDev *d_park;
     RunStats *d_run;
     hipEvent_t ev[3];
     bool ran;
+    void *d_sh;                  // SH tables of the crustal model (upload_sh)
+    size_t sh_cap;
     cor_track_point *d_trk;      // track recording of the next runs (NULL: off)
     int64_t *d_trk_off;
     EpochDev *h_epoch;           // pinned staging of set_epoch uploads
@@ -354,6 +385,7 @@ void cor_trace_plan_destroy(cor_trace_plan *p) {
     (void)hipFree(p->d_park);
     (void)hipFree(p->d_run);
     (void)hipFree(p->d_trk);
+    (void)hipFree(p->d_sh);
     (void)hipFree(p->d_trk_off);
     for (int k = 0; k < 3; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
@@ -409,6 +441,13 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     EpochDev E;
     build_epoch(models, epoch, E);
     const size_t nt = (size_t)std::max<int64_t>(total, 1);
+    {
+        const hipError_t e = upload_sh(models, &p->d_sh, &p->sh_cap, E);
+        if (e != hipSuccess) {
+            cor_trace_plan_destroy(p);
+            return fail_hip(e, "upload_sh");
+        }
+    }
 #define PLAN_TRY(call)                                    \
     do {                                                  \
         hipError_t e_ = (call);                           \
@@ -458,6 +497,9 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     // the cooperative evaluator covers IGRF and T96 (TS05 and SH traces stay per-lane)
     if (p->prec == COR_PREC_MIXED && opts->tail_lanes >= 0 && ext != EXT_TS05 && cru == CRU_NONE)
         p->coop_lanes = opts->tail_lanes == 0 ? COOP_LANES_DEFAULT : std::min(opts->tail_lanes, 64);
+    p->long_steps = 0u;
+    if (p->coop_lanes > 0 && opts->handover_steps >= 0)
+        p->long_steps = opts->handover_steps == 0 ? COOP_LONG_STEPS_DEFAULT : (uint32_t)opts->handover_steps;
 
     // persistent grid: as many resident blocks as the occupancy allows, but no more
     // blocks than are needed to give every trajectory a lane at the start (or, with the
@@ -525,6 +567,7 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.trk = p->d_trk;
     a.trk_off = p->d_trk_off;
     a.idle_weight = COOP_IDLE_WEIGHT;
+    a.long_steps = p->long_steps;
     reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop, p->d_run);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(p->ev[0], s));
@@ -637,10 +680,13 @@ int cor_trace_plan_set_epoch(cor_trace_plan *p, const cor_model_set *models, con
         HIP_TRY(hipHostMalloc((void **)&p->h_epoch, sizeof(EpochDev), hipHostMallocDefault));
         HIP_TRY(hipEventCreateWithFlags(&p->ev_up, hipEventDisableTiming));
     }
-    // the staging block may still feed the previous upload
+    // the staging block may still feed the previous upload, and the SH block the
+    // previous run
     if (p->up_pending) HIP_TRY(hipEventSynchronize(p->ev_up));
     p->up_pending = false;
+    if (p->ran) HIP_TRY(hipEventSynchronize(p->ev[2]));
     build_epoch(models, epoch, *p->h_epoch);
+    HIP_TRY(upload_sh(models, &p->d_sh, &p->sh_cap, *p->h_epoch));
     HIP_TRY(hipMemcpyAsync(p->d_epoch, p->h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s));
     if (p->coop_lanes > 0 || needs_coop_tables(p->prec, p->ext)) HIP_TRY(build_coop_tables(p->d_epoch, s));
     HIP_TRY(hipEventRecord(p->ev_up, s));
@@ -782,7 +828,10 @@ int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const 
     build_epoch(models, epoch, E);
     EpochDev *d_e = nullptr;
     double *d_in = nullptr, *d_out = nullptr;
-    hipError_t e = hipMalloc(&d_e, sizeof(EpochDev));
+    void *d_sh = nullptr;
+    size_t sh_cap = 0;
+    hipError_t e = upload_sh(models, &d_sh, &sh_cap, E);
+    if (e == hipSuccess) e = hipMalloc(&d_e, sizeof(EpochDev));
     if (e == hipSuccess) e = hipMalloc(&d_in, sizeof(double) * 3 * n);
     if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 3 * n);
     if (e == hipSuccess) e = hipMemcpyAsync(d_e, &E, sizeof(EpochDev), hipMemcpyHostToDevice, g_stream);
@@ -806,6 +855,7 @@ int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const 
     (void)hipFree(d_e);
     (void)hipFree(d_in);
     (void)hipFree(d_out);
+    (void)hipFree(d_sh);
     if (e != hipSuccess) return fail_hip(e, "cor_gpu_field");
     return COR_OK;
 }

@@ -25,7 +25,6 @@ enum { CRU_NONE = 0, CRU_SH = 1 };
 enum { ST_ALLOWED = 0, ST_FORBIDDEN_REENTRY = 1, ST_FORBIDDEN_LIMIT = 2 };
 
 constexpr int NCOEF = 105;
-constexpr int SH_NM_MAX = 24;    // SH (CHAOS-form) tables up to degree 23 on the device
 
 // Physical constants exactly as src/common.h:4-11 writes them.
 constexpr double K_PI = 3.141592654;
@@ -74,8 +73,8 @@ struct EpochDev {
     double parmod[10];
     int32_t sh_nm;               // SH table size (0 = none)
     int32_t pad;
-    double shG[SH_NM_MAX * SH_NM_MAX], shH[SH_NM_MAX * SH_NM_MAX];
-    float shGf[SH_NM_MAX * SH_NM_MAX], shHf[SH_NM_MAX * SH_NM_MAX];
+    const double *shG, *shH;     // SH tables [sh_nm][sh_nm] in their own device block
+    const float *shGf, *shHf;
     CoopTab coop;
 };
 
@@ -157,6 +156,8 @@ struct TraceArgs {
     cor_track_point *trk;        // FP64 track recording (NULL: off)
     const int64_t *trk_off;      // first point of each trajectory (n_items + 1 entries)
     long long idle_weight;       // trajectories one waiting worker is counted to take
+    uint32_t long_steps;         // per-lane steps after which a trajectory is handed over
+                                 // alone once the queue is drained (0: off)
 };
 
 }  // namespace cor

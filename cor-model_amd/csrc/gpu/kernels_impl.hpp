@@ -66,7 +66,10 @@ __global__ void __launch_bounds__(256) field_coop_kernel(FieldBatchArgs a) {
     X(INT_IGRF, EXT_TS05, CRU_NONE)        \
     X(INT_NONE, EXT_TS05, CRU_NONE)        \
     X(INT_NONE, EXT_NONE, CRU_SH)          \
-    X(INT_NONE, EXT_TS05, CRU_SH)
+    X(INT_NONE, EXT_TS05, CRU_SH)          \
+    X(INT_IGRF, EXT_NONE, CRU_SH)          \
+    X(INT_IGRF, EXT_T96, CRU_SH)           \
+    X(INT_IGRF, EXT_TS05, CRU_SH)
 
 template <int PREC>
 const void *trace_kernel_table(int intk, int ext, int cru) {

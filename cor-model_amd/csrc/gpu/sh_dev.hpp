@@ -6,8 +6,8 @@
 // one degree column: P_m^m from the sectoral chain, then the three-term recursion in n.
 // Every P and dP element is produced by the reference's own expression, so the FP64
 // values are identical; only the order of the final sum changes (the m-outer order).
-// G/H tables are [SH_NM_MAX][SH_NM_MAX] (row n, column m), read from LDS with
-// wavefront-uniform indices (broadcast reads).
+// G/H tables are [nm][nm] (row n, column m) in device memory, read with
+// wavefront-uniform indices (scalar or broadcast loads).
 //   * FP64: theta/phi by atan2 and sin/cos(m phi) per order, as the reference.
 //   * FP32: sin(theta), cos(theta) as rho/r, z/r and cos/sin(m phi) by rotation.
 #pragma once
@@ -17,7 +17,7 @@ namespace cor {
 
 constexpr double SH_RE_KM = 6371.2;   // lib/chaos/chaos.h:6
 
-template <class T, int STRIDE>
+template <class T>
 __device__ __forceinline__ v3<T> sh_geo(const T *__restrict__ G, const T *__restrict__ H, int nm, T x, T y, T z) {
     const T rho2 = x * x + y * y;
     const T rr = Dm<T>::sqrt(rho2 + z * z);
@@ -102,7 +102,7 @@ __device__ __forceinline__ v3<T> sh_geo(const T *__restrict__ G, const T *__rest
                 dp = dpn;
                 arn = arn * q;
             }
-            const T gg = G[n * STRIDE + m], hh = H[n * STRIDE + m];
+            const T gg = G[n * nm + m], hh = H[n * nm + m];
             const T t = gg * cmp + hh * smp;
             const T darn = (T)(-(n + 1)) * arn / r;
             br += t * p * darn;

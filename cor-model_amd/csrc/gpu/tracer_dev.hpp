@@ -133,7 +133,7 @@ __device__ __forceinline__ v3<double> field_fp64(const EpochView<double> &E, v3<
     if (CRU == CRU_SH) {
         // src/models.c:179-192: GSM -> GEO, synthesis, GEO -> GSM, accumulate
         const v3<double> geo = rot_gsm_to_geo(E.A, g);
-        const v3<double> bg = sh_geo<double, SH_NM_MAX>(E.shG, E.shH, E.sh_nm, geo.x, geo.y, geo.z);
+        const v3<double> bg = sh_geo<double>(E.shG, E.shH, E.sh_nm, geo.x, geo.y, geo.z);
         const v3<double> bs = rot_geo_to_gsm(E.A, bg);
         tot.x += bs.x;
         tot.y += bs.y;
@@ -168,7 +168,7 @@ __device__ __forceinline__ v3<float> field_mixed(const EpochView<float> &E, v3<d
     }
     if (CRU == CRU_SH) {
         const float x = (float)(geo_m.x / K_RE), y = (float)(geo_m.y / K_RE), z = (float)(geo_m.z / K_RE);
-        const v3<float> bg = sh_geo<float, SH_NM_MAX>(E.shG, E.shH, E.sh_nm, x, y, z);
+        const v3<float> bg = sh_geo<float>(E.shG, E.shH, E.sh_nm, x, y, z);
         tot.x += bg.x;
         tot.y += bg.y;
         tot.z += bg.z;
@@ -882,6 +882,15 @@ trace_kernel(TraceArgs a) {
                     if (live <= a.coop_lanes || idle_seen * a.idle_weight >= (long long)live) {
                         park_lanes(a, lane, active, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
                         break;
+                    }
+                    // a long-running trajectory leaves a full wavefront on its own: at
+                    // a per-lane step's latency it would be the run's last to finish
+                    if (a.long_steps) {
+                        const bool lng = active && step >= a.long_steps;
+                        if (__ballot(lng)) {
+                            park_lanes(a, lane, lng, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
+                            if (lng) active = false;
+                        }
                     }
                     idle_seen = (long long)(__hip_atomic_load(&a.coop->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
                                             __hip_atomic_load(&a.coop->res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));

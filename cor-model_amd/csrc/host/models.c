@@ -50,6 +50,21 @@ static void calc_one(CorModelKind kind, const Model *self, Vector gsm_re, Vector
 static void internal_calc(Vector gsm_pos, Vector *total, const ModelsParams *params);
 static void ts05_calc(Vector gsm_pos, Vector *total, const ModelsParams *params);
 static void t96_calc(Vector gsm_pos, Vector *total, const ModelsParams *params);
+static void chaos716_calc(Vector gsm_pos, Vector *total, const ModelsParams *params);
+
+/* The CHAOS-7.16 tables in the [186][186] layout of a model set (row n, column m, zero
+ * for m > n), expanded from the stored triangle when the library is loaded. */
+static double chaos716_g[COR_CHAOS716_NM * COR_CHAOS716_NM];
+static double chaos716_h[COR_CHAOS716_NM * COR_CHAOS716_NM];
+
+__attribute__((constructor)) static void chaos716_expand(void) {
+    int k = 0;
+    for (int n = 0; n < COR_CHAOS716_NM; n++)
+        for (int m = 0; m <= n; m++, k++) {
+            chaos716_g[n * COR_CHAOS716_NM + m] = cor_chaos716_tri[0][k];
+            chaos716_h[n * COR_CHAOS716_NM + m] = cor_chaos716_tri[1][k];
+        }
+}
 
 static const Model igrf_14 = {
     .name = "IGRF-14",
@@ -78,14 +93,30 @@ static const Model t_96 = {
     .kind = COR_MODEL_EXTERNAL_T96,
 };
 
+/* src/models.c:102-108: no input lines (setup_fun NULL) */
+static const Model chaos_7_16 = {
+    .name = "CHAOS-7.16",
+    .version = "chaos7.16",
+    .setup_data = NULL,
+    .setup_fun = NULL,
+    .calc_fun = chaos716_calc,
+    .kind = COR_MODEL_CRUSTAL_SH,
+    .sh_nm_max = COR_CHAOS716_NM,
+    .sh_g = chaos716_g,
+    .sh_h = chaos716_h,
+};
+
 static void internal_calc(Vector g, Vector *t, const ModelsParams *p) { calc_one(COR_MODEL_INTERNAL_IGRF, &igrf_14, g, t, p); }
 static void ts05_calc(Vector g, Vector *t, const ModelsParams *p) { calc_one(COR_MODEL_EXTERNAL_TS05, &ts_05, g, t, p); }
 static void t96_calc(Vector g, Vector *t, const ModelsParams *p) { calc_one(COR_MODEL_EXTERNAL_T96, &t_96, g, t, p); }
+static void chaos716_calc(Vector g, Vector *t, const ModelsParams *p) {
+    calc_one(COR_MODEL_CRUSTAL_SH, &chaos_7_16, g, t, p);
+}
 
 /* src/models.c:110-112 (the IGRF-14 chain is the internal model built here) */
 const Model *internal_models[] = {&igrf_14, NULL};
 const Model *external_models[] = {&ts_05, &t_96, NULL};
-const Model *crustal_models[] = {NULL};
+const Model *crustal_models[] = {&chaos_7_16, NULL};
 
 /* src/models.c:114-122 */
 const Model *get_model(const Model **models, const char *version) {

@@ -42,7 +42,7 @@ enum { COR_EXTERNAL_NONE = 0, COR_EXTERNAL_T96 = 1, COR_EXTERNAL_TS05 = 2 };
 enum { COR_CRUSTAL_NONE = 0, COR_CRUSTAL_SH = 1 };
 enum { COR_ALLOWED = 0, COR_FORBIDDEN_REENTRY = 1, COR_FORBIDDEN_LIMIT = 2 };
 
-#define COR_SH_NM_MAX 24   /* largest SH table (rows = cols) the device accepts */
+#define COR_SH_NM_MAX 200  /* largest SH table (rows = cols) the device accepts */
 
 /* Which evaluators the field sums, in the reference's order (internal, external,
  * crustal; src/main.c:268-299).  The SH table is [sh_nm_max][sh_nm_max], row n,
@@ -80,7 +80,10 @@ typedef struct {
                                * cooperative mode (a group of lanes per trajectory); 0 = the
                                * library default (32), negative = per-lane mode only; fuller
                                * wavefronts also hand over when a worker waits for work */
-    int32_t pad;
+    int32_t handover_steps;   /* with the cooperative tail: once the queue is drained, a
+                               * trajectory that has run this many steps in the per-lane
+                               * mode is handed over on its own, however full its
+                               * wavefront; 0 = the library default, negative = off */
 } cor_trace_opts;
 
 /* Per trajectory.  The doubles are set only for allowed trajectories: exit v/c, exit

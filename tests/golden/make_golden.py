@@ -104,21 +104,29 @@ def chaos_fixtures():
         out[f"chaos_g{k}"] = g
         out[f"chaos_h{k}"] = h
         out[f"chaos_b{k}"] = ref.chaos(g, h, pts)
+    # the shipped CHAOS-7.16 table (nm_max 186) at a subset of the points
+    out["chaos716_pts"] = pts[::8]
+    out["chaos716_b"] = ref.chaos716(out["chaos716_pts"])
     np.savez_compressed(HERE / "chaos_vectors.npz", **out)
     print("chaos vectors", {k: v.shape for k, v in out.items()})
 
 
 def field_fixtures():
-    """One field-map run of the reference CLI (-m field, src/field.c:6-54): IGRF-14 +
-    T96 on a 20 x 30 degree grid at 1.5 Re, input and output text."""
+    """Field-map runs of the reference CLI (-m field, src/field.c:6-54), input and output
+    text: IGRF-14 + T96 on a 20 x 30 degree grid at 1.5 Re, and IGRF-14 + the
+    CHAOS-7.16 crustal model on a 10 x 15 degree patch at 1.01 Re."""
     import subprocess
-    name = "field_igrf_t96"
-    inp = HERE / f"{name}.in"
-    inp.write_text("1.5 -80.0 0.0\n20.0 30.0 80.0 360.0\n2020 1 1 1 0 0 0\n-20 2 0 -2\n")
-    txt = subprocess.run([str(L.REF_BIN), "-m", "field", "-i", "igrf14", "-e", "t96", str(inp)],
-                         capture_output=True, text=True, timeout=120, check=True).stdout
-    (HERE / f"{name}.out").write_text(txt)
-    print(name, len(txt.splitlines()), "lines")
+    runs = {"field_igrf_t96": ("1.5 -80.0 0.0\n20.0 30.0 80.0 360.0\n2020 1 1 1 0 0 0\n-20 2 0 -2\n",
+                               ["-e", "t96"]),
+            "field_igrf_chaos": ("1.01 -30.0 10.0\n10.0 15.0 30.0 100.0\n2020 1 1 1 0 0 0\n",
+                                 ["-c", "chaos7.16"])}
+    for name, (text, extra) in runs.items():
+        inp = HERE / f"{name}.in"
+        inp.write_text(text)
+        txt = subprocess.run([str(L.REF_BIN), "-m", "field", "-i", "igrf14"] + extra + [str(inp)],
+                             capture_output=True, text=True, timeout=300, check=True).stdout
+        (HERE / f"{name}.out").write_text(txt)
+        print(name, len(txt.splitlines()), "lines")
 
 
 TRACK_RUNS = {

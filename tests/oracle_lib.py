@@ -229,6 +229,17 @@ class RefLib:
         pm[:len(parmod)] = parmod
         return self._pointwise(self.lib.ts05_gsm, xyz, _dp(pm))
 
+    def chaos716(self, xyz):
+        """chaos_geo with the reference's own CHAOS-7.16 table (CHAOS_7_16_COEFS)."""
+        co = C.addressof(C.c_char.in_dll(self.lib, "CHAOS_7_16_COEFS"))
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.empty_like(xyz)
+        bx, by, bz = C.c_double(), C.c_double(), C.c_double()
+        for i, (x, y, z) in enumerate(xyz):
+            self.lib.chaos_geo(x, y, z, co, C.byref(bx), C.byref(by), C.byref(bz))
+            out[i] = (bx.value, by.value, bz.value)
+        return out
+
     def chaos(self, g, h, xyz):
         """chaos_geo (lib/chaos/chaos.c:120-227) with a caller's [nm][nm] G/H table."""
 

@@ -58,11 +58,11 @@ def lib():
     return L_
 
 
-def registry_models(L_, internal, external):
-    """NULL-terminated model list as src/main.c builds it from -i / -e."""
-    models = (C.c_void_p * 3)()
+def registry_models(L_, internal, external, crustal=None):
+    """NULL-terminated model list as src/main.c builds it from -i / -e / -c."""
+    models = (C.c_void_p * 4)()
     k = 0
-    for reg, ver in (("internal_models", internal), ("external_models", external)):
+    for reg, ver in (("internal_models", internal), ("external_models", external), ("crustal_models", crustal)):
         if ver:
             m = L_.get_model(C.addressof(C.c_void_p.in_dll(L_, reg)), ver.encode())
             assert m, ver
@@ -72,11 +72,14 @@ def registry_models(L_, internal, external):
 
 
 def setup_models(models, f, header, params):
-    """src/main.c:51-62: every model's setup_fun reads its lines of the input file."""
+    """src/main.c:51-62: every model's setup_fun reads its lines of the input file
+    (models without one, like the crustal model, are skipped)."""
     for m in models:
         if not m:
             break
         md = Model.from_address(m)
+        if not C.cast(md.setup_fun, C.c_void_p).value:
+            continue
         assert md.setup_fun(f, C.addressof(header), C.addressof(params), md.setup_data) == 0
 
 
@@ -156,14 +159,17 @@ def test_trajectory_simulation_cutoff_text(tmp_path, name, internal, external, s
     print(name, n, "lines compared column by column")
 
 
-def test_field_simulation_text(tmp_path):
+FIELD_RUNS = [("field_igrf_t96", "t96", None, 9 * 12), ("field_igrf_chaos", None, "chaos7.16", 7 * 6)]
+
+
+@pytest.mark.parametrize("name,external,crustal,n_points", FIELD_RUNS, ids=[r[0] for r in FIELD_RUNS])
+def test_field_simulation_text(tmp_path, name, external, crustal, n_points):
     A.init(0)
     L_ = lib()
-    name = "field_igrf_t96"
     inp = GOLD / f"{name}.in"
     h = InfileHeader()
     p = A.ModelsParams()
-    models = registry_models(L_, "igrf14", "t96")
+    models = registry_models(L_, "igrf14", external, crustal)
     f = libc.fopen(str(inp).encode(), b"r")
     try:
         assert L_.read_field_infile_header(C.c_void_p(f), C.byref(h)) == 0
@@ -193,7 +199,7 @@ def test_field_simulation_text(tmp_path):
             continue
         assert va.shape == vb.shape and np.all(np.abs(va - vb) <= 0.0100001), (a, b)
         nums += 1
-    assert nums == 9 * 12
+    assert nums == n_points
 
 
 TRACK_RUNS = [("track_t96_5gv", "t96", 20000), ("track_t96_trapped", "t96", 3000), ("track_ts05_3gv", "ts05", 20000)]

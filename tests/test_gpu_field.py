@@ -178,3 +178,29 @@ def test_field_sh(cor, oracle, name, ext, parmod):
     err = np.linalg.norm(got - want, axis=1) / scale
     print(name, "mixed max rel err", err.max(), "p99", np.quantile(err, 0.99))
     assert np.all(np.isfinite(got)) and err.max() < 1e-3 and np.quantile(err, 0.99) < 1e-4
+
+
+def test_field_chaos716(cor, oracle):
+    """The registry's CHAOS-7.16 crustal model (degree 185, lib/chaos/chaos.c:118-227
+    with lib/chaos/coefs/chaos_7_16.h) near the surface: FP64 within 1e-10 of the
+    crustal field alone (with a 1e-6 nT floor); IGRF + CHAOS-7.16 mixed at the FP32 field
+    bar."""
+    from test_host_lib import chaos716_tables
+    _, g, h = chaos716_tables()
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    pts = points(21, 300, rmin=1.0, rmax=1.5)
+    want = oracle.field(L.models(internal=0, crustal=1, chaos=(g, h)), po, pts)
+    got = cor.field(cor.model_set(cor.INTERNAL_NONE, sh=(g, h)), p, pts, cor.PREC_FP64)
+    # relative to |B| with a 1e-6 nT floor (the degree-21+ field falls as r^-23 and is
+    # about 1e-3 nT at 1.5 Re)
+    err = np.linalg.norm(got - want, axis=1) / (np.linalg.norm(want, axis=1) + 1e-6)
+    print("chaos7.16 fp64 max rel err", err.max(), "|B| median", np.median(np.linalg.norm(want, axis=1)))
+    assert np.all(np.isfinite(got)) and err.max() < 1e-10
+    want = oracle.field(L.models(internal=1, crustal=1, chaos=(g, h)), po, pts)
+    got = cor.field(cor.model_set(cor.INTERNAL_IGRF, sh=(g, h)), p, pts, cor.PREC_MIXED)
+    r = np.linalg.norm(pts, axis=1) / RE
+    scale = np.linalg.norm(want, axis=1) + 30574.0 / r ** 3
+    err = np.linalg.norm(got - want, axis=1) / scale
+    print("igrf+chaos7.16 mixed max rel err", err.max(), "p99", np.quantile(err, 0.99))
+    assert np.all(np.isfinite(got)) and err.max() < 1e-3 and np.quantile(err, 0.99) < 1e-4

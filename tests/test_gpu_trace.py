@@ -136,19 +136,28 @@ def test_t96_sites_cooperative(cor, oracle, lat, lon):
 def test_cooperative_handover_many_scans(cor):
     """Many scans through the handover: every trajectory finishes exactly once (status
     set, steps counted) whichever mode ran it, and the per-scan cutoffs of the per-lane
-    and cooperative runs agree within the penumbra noise."""
+    and cooperative runs agree within the penumbra noise.  Cases: per-lane only; sparse
+    wavefronts handed over (8 lanes); every wavefront at the drain (64); and single
+    trajectories leaving full wavefronts after 2 000 per-lane steps."""
     sites = cor.grid_sites()[::97]
     rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
     out = {}
-    for tl in (-1, 8, 64):
-        tr, cut = cor.trace(ms, p, rows, step_limit=200000, precision=cor.PREC_MIXED, tail_lanes=tl)
+    cases = [(-1, -1), (8, -1), (64, -1), (8, 2000)]
+    for tl, hs in cases:
+        tr, cut = cor.trace(ms, p, rows, step_limit=200000, precision=cor.PREC_MIXED, tail_lanes=tl,
+                            handover_steps=hs)
         assert np.all(np.isin(tr["status"], (0, 1, 2))) and np.all(tr["steps"] >= 1)
-        out[tl] = cut
-    for tl in (8, 64):
-        d = np.abs(out[tl]["rc"] - out[-1]["rc"])
-        print(tl, "max |dRc|", d.max(), "frac <= 0.1 GV", (d <= 0.1 + 1e-9).mean())
+        if hs > 0:
+            # a trajectory handed over on its own left at its 2 000th step or later
+            ho = tr["handover"][tr["handover"] > 0]
+            print("handed over", len(ho), "min handover step", ho.min() if len(ho) else None)
+            assert len(ho) > 0
+        out[(tl, hs)] = cut
+    for key in cases[1:]:
+        d = np.abs(out[key]["rc"] - out[(-1, -1)]["rc"])
+        print(key, "max |dRc|", d.max(), "frac <= 0.1 GV", (d <= 0.1 + 1e-9).mean())
         assert (d <= 0.1 + 1e-9).mean() >= 0.8
 
 

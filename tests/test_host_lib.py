@@ -185,3 +185,37 @@ def test_refine_scan():
     assert np.isclose(f["rig"], 19.99) and f["rk"] == 20.0 and A.scan_count(f) == 11
     with pytest.raises(A.CorError):
         A.refine_scans(row, cut, 0)
+
+
+def chaos716_tables():
+    """The crustal registry's CHAOS-7.16 tables as the library holds them ([186][186])."""
+    lib = A.lib()
+
+    class Model(C.Structure):
+        _fields_ = [("name", C.c_char_p), ("version", C.c_char_p), ("setup_data", C.c_void_p),
+                    ("setup_fun", C.c_void_p), ("calc_fun", C.c_void_p), ("kind", C.c_int),
+                    ("sh_nm_max", C.c_int32), ("sh_g", C.POINTER(C.c_double)), ("sh_h", C.POINTER(C.c_double))]
+
+    lib.get_model.restype = C.c_void_p
+    lib.get_model.argtypes = [C.c_void_p, C.c_char_p]
+    m = lib.get_model(C.addressof(C.c_void_p.in_dll(lib, "crustal_models")), b"chaos7.16")
+    assert m
+    md = Model.from_address(m)
+    nm = md.sh_nm_max
+    g = np.ctypeslib.as_array(md.sh_g, shape=(nm, nm)).copy()
+    h = np.ctypeslib.as_array(md.sh_h, shape=(nm, nm)).copy()
+    return md.name.decode(), g, h
+
+
+def test_chaos716_table_matches_the_reference():
+    """The CHAOS-7.16 coefficients the library carries (extracted by gen_chaos_table.py)
+    reproduce the reference's chaos_geo with its own table bit for bit, through the
+    oracle's synthesis (itself bit-identical to chaos_geo on seeded tables)."""
+    name, g, h = chaos716_tables()
+    assert name == "CHAOS-7.16" and g.shape == (186, 186)
+    assert np.all(np.triu(g, 1) == 0) and np.all(np.triu(h, 1) == 0) and np.all(h[:, 0] == 0)
+    cv = np.load(GOLD / "chaos_vectors.npz")
+    got = L.Oracle().chaos(g, h, cv["chaos716_pts"])
+    want = cv["chaos716_b"]
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), np.argwhere(~same)[:5]
