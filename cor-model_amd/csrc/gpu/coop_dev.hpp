@@ -298,12 +298,14 @@ __device__ __forceinline__ void wave_sum(float *p) {
     for (int k = 0; k < N; k++) p[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[k]), 63));
 }
 
-// Sum of each of N values over a group of G lanes (G = 16: the four DPP row steps; every
-// lane of the row ends with the same bits, since each step adds two values that the
-// partner lane adds in the other order).  G = 64: wave_sum.
+// Sum of each of N values over a group of G lanes (G = 8: the three DPP steps within a
+// half row; G = 16: the four row steps; G = 32: the row sums plus the partner row's,
+// read by a lane permute; every lane of the group ends with the same bits, since each
+// step adds two values that the partner lane adds in the other order).  G = 64:
+// wave_sum.
 template <int N, int G>
 __device__ __forceinline__ void group_sum(float *p) {
-    static_assert(G == 16 || G == 64, "groups of 16 or 64 lanes");
+    static_assert(G == 8 || G == 16 || G == 32 || G == 64, "groups of 8, 16, 32 or 64 lanes");
     if constexpr (G == 64) {
         wave_sum<N>(p);
     } else {
@@ -313,8 +315,14 @@ __device__ __forceinline__ void group_sum(float *p) {
         for (int k = 0; k < N; k++) p[k] = dpp_add<0x4E, 0xF>(p[k]);    // quad_perm [2,3,0,1]
 #pragma unroll
         for (int k = 0; k < N; k++) p[k] = dpp_add<0x141, 0xF>(p[k]);   // row_half_mirror
+        if constexpr (G >= 16) {
 #pragma unroll
-        for (int k = 0; k < N; k++) p[k] = dpp_add<0x140, 0xF>(p[k]);   // row_mirror
+            for (int k = 0; k < N; k++) p[k] = dpp_add<0x140, 0xF>(p[k]);   // row_mirror
+        }
+        if constexpr (G == 32) {
+#pragma unroll
+            for (int k = 0; k < N; k++) p[k] = p[k] + __shfl_xor(p[k], 16);
+        }
     }
 }
 
@@ -333,15 +341,16 @@ __device__ __forceinline__ void put4(float *d, float4 v) {
     d[3] = v.w;
 }
 
-// gl: the lane's index in its group.  With groups of 16 the Cartesian harmonics and the
-// current loops (more terms than lanes) are read in their passes instead.
+// gl: the lane's index in its group.  Families with more terms than the group has
+// lanes (groups of 16: the Cartesian harmonics and the current loops; groups of 8 also
+// the conical harmonics and the Region-2 sheet) are read in their passes instead.
 template <int G>
 __device__ __forceinline__ void load_lane(const CoopTab &T, int gl, CoopLane &L) {
     if constexpr (G == 64) L.harm = ld4(T.harm[gl]);
     L.cone = ld4(T.cone[gl & 7]);
     L.cyl = ld4(T.cyl[gl & 7]);
     L.t87 = ld4(T.t87[gl & 3]);
-    L.conic = ld4(T.conic[gl & 15]);
+    if constexpr (G >= 16) L.conic = ld4(T.conic[gl & 15]);
 #pragma unroll
     for (int j = 0; j < 6; j++) put4(L.icon + 4 * j, ld4(&T.icon[gl & 3][4 * j]));
     if constexpr (G == 64) {
@@ -349,8 +358,10 @@ __device__ __forceinline__ void load_lane(const CoopTab &T, int gl, CoopLane &L)
 #pragma unroll
         for (int j = 0; j < 4; j++) put4(L.circ + 4 * j, ld4(&T.circ[ci][4 * j]));
     }
+    if constexpr (G >= 16) {
 #pragma unroll
-    for (int j = 0; j < 6; j++) put4(L.sheet + 4 * j, ld4(&T.sheet[gl & 15][4 * j]));
+        for (int j = 0; j < 6; j++) put4(L.sheet + 4 * j, ld4(&T.sheet[gl & 15][4 * j]));
+    }
 }
 
 // One term of the harmonic form shared by the shield sets, birk1shld, birk2shl and the
@@ -648,9 +659,13 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
             pz += k * (c[2] * bl.x + c[5] * bl.y + c[8] * bl.z);
         }
     }
-    // ---- family 8: conical harmonics (10 lanes: Region-1 conical basis, Region-2 inner)
-    if (gl < CO_NCONIC) {
-        const float4 c = L.conic;
+    // ---- family 8: conical harmonics (10 terms: Region-1 conical basis, Region-2 inner;
+    // one pass, or two with groups of 8)
+#pragma unroll
+    for (int pass = 0; pass < (CO_NCONIC + G - 1) / G; pass++) {
+        const int t = gl + G * pass;
+        if (t >= CO_NCONIC) break;
+        const float4 c = G >= 16 ? L.conic : ld4(T.conic[t]);
         const bool g0 = c.w == F(0.0);
         const F w = g0 ? wc : wi;
         if (w != F(0.0)) {
@@ -686,9 +701,17 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
             pz += k * (-bt * s);
         }
     }
-    // ---- family 9: Region-2 sheet (15 lanes)
-    if (ws != F(0.0) && gl < CO_NSHEET) {
-        const float *c = L.sheet;
+    // ---- family 9: Region-2 sheet (15 terms; one pass, or two with groups of 8)
+#pragma unroll
+    for (int pass = 0; pass < (CO_NSHEET + G - 1) / G; pass++) {
+        const int t = gl + G * pass;
+        if (ws == F(0.0) || t >= CO_NSHEET) break;
+        float sbuf[24];
+        if constexpr (G < 16) {
+#pragma unroll
+            for (int jj = 0; jj < 6; jj++) put4(sbuf + 4 * jj, ld4(&T.sheet[t][4 * jj]));
+        }
+        const float *c = G >= 16 ? L.sheet : sbuf;
         const int comp = (int)c[3];
         F t1, t2, t3;
         tfac(xks, c[4], c[5], c[6], c[7], &t1, &t2, &t3);
@@ -740,15 +763,17 @@ __device__ __forceinline__ v3<float> t96_finish(const T96Share &S, v3<float> fx)
 
 // ---------------------------------------------------------------- IGRF, cooperative
 
-// lib/internal/internal.c:182-275 with one order m per lane (lanes 0..kw-1): each lane
-// advances the sectoral terms to its order, then runs the degree recursion of its
-// column.  Returns this lane's share of (Br, Btheta, Bphi*sin).
+// lib/internal/internal.c:182-275 with one order m per lane (lanes 0..kw-1; groups of 8
+// take a second order, m + 8, in a second pass): each lane advances the sectoral terms
+// to its order, then runs the degree recursion of its column.  Returns this lane's share
+// of (Br, Btheta, Bphi*sin).
 struct IgrfShare {
     float br, bt, bf;
     float s, c, cf, sf;
     bool polar;
 };
 
+template <int GL>
 __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G, const float *__restrict__ H,
                                                      const float *__restrict__ REC, float x, float y, float z,
                                                      int gl) {
@@ -770,46 +795,53 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
     int nm = 3 + 30 / ((int)r + 2);
     if (nm > 13) nm = 13;
     const int kw = nm + 1;
-    const int m = gl + 1;     // groups of at least 16 lanes: one order each (at most 14)
     F am = inv_r * inv_r, pmm = F(1.), dmm = F(0.), cm = F(0.), sm = F(1.);
-    for (int j = 1; j < kw; j++) {
-        if (j < m) {
-            const F w = cm;
-            cm = w * cf + sm * sf;
-            sm = sm * cf - w * sf;
-            const F d2 = s * dmm + c * pmm;
-            pmm = s * pmm;
-            dmm = d2;
-            am = am * inv_r;
+    F br = F(0.), bt = F(0.), bf = F(0.);
+    constexpr int passes = GL >= 16 ? 1 : 2;      // at most 14 orders
+#pragma unroll
+    for (int pass = 0; pass < passes; pass++) {
+        const int m = gl + 1 + GL * pass;
+        const int m0 = pass == 0 ? 1 : m - GL;    // order the sectoral terms are at
+        for (int j = 1; j < kw; j++) {
+            if (j >= m0 && j < m) {
+                const F w = cm;
+                cm = w * cf + sm * sf;
+                sm = sm * cf - w * sf;
+                const F d2 = s * dmm + c * pmm;
+                pmm = s * pmm;
+                dmm = d2;
+                am = am * inv_r;
+            }
         }
-    }
-    F br = F(0.), bt = F(0.), bphi = F(0.);
-    if (m <= kw) {
-        F q = pmm, dq = dmm, q2 = F(0.), dq2 = F(0.), an = am;
-        int idx = m * (m - 1) / 2 + m - 1;
-        F g = G[idx], h = H[idx], rec = REC[idx];
-        for (int n = m; n <= kw; n++) {
-            const F gn = G[idx + n], hn = H[idx + n], recn = REC[idx + n];   // tables padded to 128
-            const F w = g * sm + h * cm;
-            br = br + (an * n) * w * q;
-            bt = bt - an * w * dq;
-            bphi = bphi + an * (g * cm - h * sm) * (S.polar ? dq : q);
-            const F dnext = c * dq - s * q - rec * dq2;
-            const F pnext = c * q - rec * q2;
-            dq2 = dq;
-            q2 = q;
-            dq = dnext;
-            q = pnext;
-            an = an * inv_r;
-            idx += n;
-            g = gn;
-            h = hn;
-            rec = recn;
+        if (m <= kw) {
+            F bphi = F(0.);
+            F q = pmm, dq = dmm, q2 = F(0.), dq2 = F(0.), an = am;
+            int idx = m * (m - 1) / 2 + m - 1;
+            F g = G[idx], h = H[idx], rec = REC[idx];
+            for (int n = m; n <= kw; n++) {
+                const F gn = G[idx + n], hn = H[idx + n], recn = REC[idx + n];   // tables padded to 128
+                const F w = g * sm + h * cm;
+                br = br + (an * n) * w * q;
+                bt = bt - an * w * dq;
+                bphi = bphi + an * (g * cm - h * sm) * (S.polar ? dq : q);
+                const F dnext = c * dq - s * q - rec * dq2;
+                const F pnext = c * q - rec * q2;
+                dq2 = dq;
+                q2 = q;
+                dq = dnext;
+                q = pnext;
+                an = an * inv_r;
+                idx += n;
+                g = gn;
+                h = hn;
+                rec = recn;
+            }
+            if (m >= 2) bf = bf + bphi * (F)(m - 1);
         }
     }
     S.br = br;
     S.bt = bt;
-    S.bf = m >= 2 ? bphi * (F)(m - 1) : F(0.);
+    S.bf = bf;
     return S;
 }
 
@@ -834,13 +866,13 @@ template <int INT, int EXT, int GS>
 __device__ __forceinline__ v3<float> coop_field(const float *G, const float *H, const float *REC, const float *Af,
                                                 const T96Params<float> &P, const CoopTab &T, v3<double> geo_m,
                                                 v3<double> gsm_m, int gl) {
-    static_assert(GS >= 16, "one IGRF order per lane needs 14 lanes");
+    static_assert(GS == 8 || GS == 16 || GS == 32 || GS == 64, "groups of 8, 16, 32 or 64 lanes");
     typedef float F;
     IgrfShare I;
     T96Share X;
     F p[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (INT == INT_IGRF) {
-        I = coop_igrf_share(G, H, REC, (F)(geo_m.x / K_RE), (F)(geo_m.y / K_RE), (F)(geo_m.z / K_RE), gl);
+        I = coop_igrf_share<GS>(G, H, REC, (F)(geo_m.x / K_RE), (F)(geo_m.y / K_RE), (F)(geo_m.z / K_RE), gl);
         p[0] = I.br;
         p[1] = I.bt;
         p[2] = I.bf;

@@ -810,11 +810,12 @@ int cor_gpu_trace_tracks(const cor_model_set *models, const ModelsParams *epoch,
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
                   double *geo_nT, int64_t n, int32_t precision) {
     if (!epoch || n < 0 || (n > 0 && (!gsm_m || !geo_nT))) return COR_E_INVALID;
-    if (precision != COR_PREC_FP64 && precision != COR_PREC_MIXED && precision != COR_PREC_MIXED_WAVE &&
-        precision != COR_PREC_MIXED_GROUP)
-        return COR_E_INVALID;
-    const bool coop = precision == COR_PREC_MIXED_WAVE || precision == COR_PREC_MIXED_GROUP;
-    const int lanes = precision == COR_PREC_MIXED_GROUP ? 16 : 64;
+    if (precision < COR_PREC_FP64 || precision > COR_PREC_MIXED_GROUP32) return COR_E_INVALID;
+    const bool coop = precision >= COR_PREC_MIXED_WAVE;
+    const int lanes = precision == COR_PREC_MIXED_GROUP     ? COOP_GROUP
+                      : precision == COR_PREC_MIXED_GROUP16 ? 16
+                      : precision == COR_PREC_MIXED_GROUP32 ? 32
+                                                            : 64;
     int rc = ensure_init();
     if (rc) return rc;
     int intk, ext, cru;

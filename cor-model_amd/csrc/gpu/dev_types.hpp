@@ -90,6 +90,10 @@ struct ParkDev {
     uint32_t ready;              // = run id once the record is complete
 };
 
+// Lanes per trajectory when a cooperative worker runs several trajectories (eight to a
+// wavefront); a lone trajectory gets all 64.
+constexpr int COOP_GROUP = 8;
+
 // Work-handover counters of one run (cleared by the reset kernel).
 // The counters each sit in a cache line of their own: workers poll them while others
 // update them.

@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(256) field_batch_kernel(FieldBatchArgs a) {
 }
 
 // The same batch with the cooperative evaluator as the tracer's cooperative mode runs
-// it: one group of GS lanes (16 or 64) per point, points strided over the grid's groups.
+// it: one group of GS lanes (COOP_GROUP or 64) per point, points strided over the grid's groups.
 // Every lane of a wavefront takes part in each pass (the group sums need all lanes);
 // groups past the end evaluate the last point again and write nothing.
 template <int INT, int EXT, int CRU, int GS>

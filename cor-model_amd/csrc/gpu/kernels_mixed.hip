@@ -10,7 +10,12 @@ const void *field_kernel_mixed(int intk, int ext, int cru) {
     return field_kernel_table<PREC_MIXED>(intk, ext, cru);
 }
 const void *field_kernel_coop(int intk, int ext, int cru, int lanes) {
-    return lanes == 16 ? field_coop_table<16>(intk, ext, cru) : field_coop_table<64>(intk, ext, cru);
+    switch (lanes) {
+    case 64: return field_coop_table<64>(intk, ext, cru);
+    case 32: return field_coop_table<32>(intk, ext, cru);
+    case 16: return field_coop_table<16>(intk, ext, cru);
+    default: return field_coop_table<COOP_GROUP>(intk, ext, cru);
+    }
 }
 
 __global__ void build_coop_kernel(EpochDev *E) {

@@ -578,10 +578,34 @@ __device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool ac
     }
 }
 
-// Cooperative worker.  A wavefront runs up to four parked trajectories, one per group of
-// 16 lanes, while parked work is plentiful, and a lone trajectory with all 64 lanes
-// (its step then takes about a third of the time); groups of a wavefront hold the same
-// state in all their lanes and compute the same bits.
+// The field of a wavefront's few trajectories with FG lanes each (fewer, wider groups
+// than the bookkeeping groups of COOP_GROUP lanes: a wavefront step costs about the same
+// whatever the number of trajectories, and wider groups finish it sooner).  Field group
+// j evaluates the j-th live trajectory (leads: the first lanes of the bookkeeping groups
+// that hold one); each bookkeeping group then reads its trajectory's field from that
+// field group, so all its lanes hold the same bits.
+template <int INT, int EXT, int FG>
+__device__ __forceinline__ v3<float> coop_field_spread(const EpochView<float> &E, const CoopTab &T, uint64_t leads,
+                                                      v3<double> pos, int lane, int lead) {
+    const int fg = lane / FG;
+    uint64_t m = leads;
+#pragma unroll
+    for (int j = 0; j < 64 / FG - 1; j++)
+        if (j < fg) m &= m - 1ull;
+    const int src = (m ? __ffsll((unsigned long long)m) : __ffsll((unsigned long long)leads)) - 1;
+    const v3<double> p1 = mk3(__shfl(pos.x, src), __shfl(pos.y, src), __shfl(pos.z, src));
+    const v3<double> gsm = rot_geo_to_gsm(E.A, p1);
+    const v3<float> fo = coop_field<INT, EXT, FG>(E.G, E.H, E.REC, E.Af, E.t96, T, p1, gsm, lane & (FG - 1));
+    const int r = __popcll(leads & ((1ull << lead) - 1ull));
+    const int from = (r < 64 / FG ? r : 0) * FG;
+    return mk3(__shfl(fo.x, from), __shfl(fo.y, from), __shfl(fo.z, from));
+}
+
+// Cooperative worker.  A wavefront runs up to eight parked trajectories, one per group of
+// COOP_GROUP lanes, while parked work is plentiful; with four or fewer the field is
+// evaluated with wider groups (a lone trajectory with all 64 lanes: its step then takes
+// about two thirds of the time).  Groups of a wavefront hold the same state in all
+// their lanes and compute the same bits.
 //   * An idle wavefront takes one ticket for the next park slot (tickets are served in
 //     slot order) and waits until the slot is reserved and published.
 //   * A busy wavefront takes tickets for its free groups only for slots already
@@ -599,7 +623,7 @@ __device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool ac
 // wait.
 template <int INT, int EXT>
 __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const CoopTab &T, int lane) {
-    constexpr int GS = 16;
+    constexpr int GS = COOP_GROUP;
     const int gl = lane & (GS - 1);
     const int lead = lane & ~(GS - 1);            // first lane of this lane's group
     const bool is_lead = gl == 0;
@@ -751,14 +775,20 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
                 }
             }
         }
-        // ---- 3. one step of each trajectory: a lone one with all 64 lanes, several with
-        // a group of 16 lanes each (groups without one evaluate at their last point)
+        // ---- 3. one step of each trajectory.  The field: a lone trajectory with all 64
+        // lanes, two with 32 lanes each, three or four with 16, more with their own
+        // group of COOP_GROUP lanes (groups without one evaluate at their last point).
         v3<float> f;
-        if (__popcll(has_leads) == 1) {
+        const int k_live = __popcll(has_leads);
+        if (k_live == 1) {
             const int src = __ffsll((unsigned long long)has_leads) - 1;
             const v3<double> p1 = mk3(__shfl(pos.x, src), __shfl(pos.y, src), __shfl(pos.z, src));
             const v3<double> gsm = rot_geo_to_gsm(E.A, p1);
             f = coop_field<INT, EXT, 64>(E.G, E.H, E.REC, E.Af, E.t96, T, p1, gsm, lane);
+        } else if (k_live == 2) {
+            f = coop_field_spread<INT, EXT, 32>(E, T, has_leads, pos, lane, lead);
+        } else if (k_live <= 4) {
+            f = coop_field_spread<INT, EXT, 16>(E, T, has_leads, pos, lane, lead);
         } else {
             const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
             f = coop_field<INT, EXT, GS>(E.G, E.H, E.REC, E.Af, E.t96, T, pos, gsm, gl);

@@ -33,10 +33,12 @@ enum {
     COR_E_NOMEM = -5
 };
 
-/* COR_PREC_MIXED_WAVE / COR_PREC_MIXED_GROUP: cor_gpu_field only -- the mixed-precision
- * model evaluated as the tracer's cooperative mode evaluates it, one wavefront (64 lanes)
- * or one group of 16 lanes per point. */
-enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1, COR_PREC_MIXED_WAVE = 2, COR_PREC_MIXED_GROUP = 3 };
+/* COR_PREC_MIXED_WAVE / _GROUP / _GROUP16 / _GROUP32: cor_gpu_field only -- the
+ * mixed-precision model evaluated as the tracer's cooperative mode evaluates it: one
+ * wavefront (64 lanes) per point, one group of 8 lanes (eight trajectories to a
+ * wavefront), or 16 or 32 lanes (four or two to a wavefront). */
+enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1, COR_PREC_MIXED_WAVE = 2, COR_PREC_MIXED_GROUP = 3,
+       COR_PREC_MIXED_GROUP16 = 4, COR_PREC_MIXED_GROUP32 = 5 };
 enum { COR_INTERNAL_NONE = 0, COR_INTERNAL_IGRF = 1 };
 enum { COR_EXTERNAL_NONE = 0, COR_EXTERNAL_T96 = 1, COR_EXTERNAL_TS05 = 2 };
 enum { COR_CRUSTAL_NONE = 0, COR_CRUSTAL_SH = 1 };

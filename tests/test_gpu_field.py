@@ -80,19 +80,20 @@ def test_field_mixed_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
     assert np.quantile(err, 0.99) < 1e-4
 
 
-@pytest.mark.parametrize("lanes", [64, 16])
+@pytest.mark.parametrize("lanes", [64, 32, 16, 8])
 @pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
 def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts, lanes):
-    """The tracer's cooperative evaluator (one wavefront, or one group of 16 lanes, per
-    point; term families in SIMD passes; one cross-lane sum): same FP32 bar as the
-    per-lane mixed path.  With groups of 16, the four points of a wavefront are
+    """The tracer's cooperative evaluator (one wavefront, or one group of 32, 16 or 8
+    lanes, per point; term families in SIMD passes; one cross-lane sum): same FP32 bar
+    as the per-lane mixed path.  With smaller groups, the points of a wavefront are
     different points, so the groups follow different model branches."""
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
     _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
     if parmod:
         for k, v in enumerate(parmod):
             po.PARMOD[k] = v
-    prec = cor.PREC_MIXED_WAVE if lanes == 64 else cor.PREC_MIXED_GROUP
+    prec = {64: cor.PREC_MIXED_WAVE, 32: cor.PREC_MIXED_GROUP32, 16: cor.PREC_MIXED_GROUP16,
+            8: cor.PREC_MIXED_GROUP}[lanes]
     got = cor.field(cor.model_set(**kinds), p, pts, prec)
     want = oracle.field(L.models(**kinds), po, pts)
     r = np.linalg.norm(pts, axis=1) / RE
