@@ -872,13 +872,14 @@ __device__ __forceinline__ v3<float> coop_field(const float *G, const float *H, 
     T96Share X;
     F p[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (INT == INT_IGRF) {
-        I = coop_igrf_share<GS>(G, H, REC, (F)(geo_m.x / K_RE), (F)(geo_m.y / K_RE), (F)(geo_m.z / K_RE), gl);
+        I = coop_igrf_share<GS>(G, H, REC, (F)(geo_m.x * K_INV_RE), (F)(geo_m.y * K_INV_RE), (F)(geo_m.z * K_INV_RE),
+                                gl);
         p[0] = I.br;
         p[1] = I.bt;
         p[2] = I.bf;
     }
     if constexpr (EXT == EXT_T96) {
-        X = coop_t96_share<GS>(P, T, (F)(gsm_m.x / K_RE), (F)(gsm_m.y / K_RE), (F)(gsm_m.z / K_RE), gl);
+        X = coop_t96_share<GS>(P, T, (F)(gsm_m.x * K_INV_RE), (F)(gsm_m.y * K_INV_RE), (F)(gsm_m.z * K_INV_RE), gl);
         p[3] = X.fx.x;
         p[4] = X.fx.y;
         p[5] = X.fx.z;

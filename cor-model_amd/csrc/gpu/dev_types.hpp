@@ -30,6 +30,7 @@ constexpr int NCOEF = 105;
 constexpr double K_PI = 3.141592654;
 constexpr double K_2PI = 6.28318531;
 constexpr double K_RE = 6.3712e+06;
+constexpr double K_INV_RE = 1.0 / K_RE;   // mixed path: positions in Re for the FP32 field
 constexpr double K_TO_RAD = K_PI / 180.;
 constexpr double K_Q = 1.6021e-19;
 constexpr double K_C = 2.99725e+08;

@@ -143,31 +143,32 @@ __device__ __forceinline__ v3<double> field_fp64(const EpochView<double> &E, v3<
 }
 
 // Mixed path: the SH model takes the GEO position directly (no GSM round trip), the
-// external model's GSM field is rotated to GEO in FP32.
+// external model's GSM field is rotated to GEO in FP32.  Positions go to Re by a
+// multiplication (the FP32 argument is the same to within its last bit).
 template <int INT, int EXT, int CRU>
 __device__ __forceinline__ v3<float> field_mixed(const EpochView<float> &E, v3<double> geo_m,
                                                  v3<double> gsm_m) {
     v3<float> tot = mk3(0.f, 0.f, 0.f);
     if (INT == INT_IGRF) {
-        const float x = (float)(geo_m.x / K_RE), y = (float)(geo_m.y / K_RE), z = (float)(geo_m.z / K_RE);
+        const float x = (float)(geo_m.x * K_INV_RE), y = (float)(geo_m.y * K_INV_RE), z = (float)(geo_m.z * K_INV_RE);
         tot = igrf_geo<float>(E.G, E.H, E.REC, x, y, z);
     }
     if (EXT == EXT_T96) {
-        const float x = (float)(gsm_m.x / K_RE), y = (float)(gsm_m.y / K_RE), z = (float)(gsm_m.z / K_RE);
+        const float x = (float)(gsm_m.x * K_INV_RE), y = (float)(gsm_m.y * K_INV_RE), z = (float)(gsm_m.z * K_INV_RE);
         const v3<float> bg = rot_gsm_to_geo_f(E.Af, t96_field<float>(E.t96, x, y, z, E.icon));
         tot.x += bg.x;
         tot.y += bg.y;
         tot.z += bg.z;
     }
     if (EXT == EXT_TS05) {
-        const float x = (float)(gsm_m.x / K_RE), y = (float)(gsm_m.y / K_RE), z = (float)(gsm_m.z / K_RE);
+        const float x = (float)(gsm_m.x * K_INV_RE), y = (float)(gsm_m.y * K_INV_RE), z = (float)(gsm_m.z * K_INV_RE);
         const v3<float> bg = rot_gsm_to_geo_f(E.Af, ts05_field<float>(*E.ts05, *E.ts05d, x, y, z));
         tot.x += bg.x;
         tot.y += bg.y;
         tot.z += bg.z;
     }
     if (CRU == CRU_SH) {
-        const float x = (float)(geo_m.x / K_RE), y = (float)(geo_m.y / K_RE), z = (float)(geo_m.z / K_RE);
+        const float x = (float)(geo_m.x * K_INV_RE), y = (float)(geo_m.y * K_INV_RE), z = (float)(geo_m.z * K_INV_RE);
         const v3<float> bg = sh_geo<float>(E.shG, E.shH, E.sh_nm, x, y, z);
         tot.x += bg.x;
         tot.y += bg.y;
@@ -228,57 +229,39 @@ __device__ __forceinline__ void rk_step_fp64(double h, double d, v3<double> B, v
     pos.z += (23. * p[2][0] + 125. * p[2][2] - 81. * p[2][4] + 125. * p[2][5]) / 192.;
 }
 
-// Same tableau with the stage arithmetic in FP32: only the increments dv, dx are formed
-// (in FP32, relative to the FP64 velocity), and they are added to the FP64 state.
-// Tableau divisions become multiplications by the FP32 reciprocals.
+// The same step in FP32 increments added to the FP64 state, in closed form.  With the
+// field frozen over the step the velocity stages are linear in v: k_s = hd (u_s x B),
+// u_s = v + sum_j a_sj k_j.  The tableau's stability polynomial is exactly
+// 1 + z + z^2/2 + z^3/6 + z^4/24 + z^5/120 (no z^6 term), and for z = hd [. x B] the
+// powers reduce to two vectors, c = v x B and e = c x B (z^3 v = -hd^2 |B|^2 z v, ...):
+//   dv = hd (1 - s/6 + s^2/120) c + hd^2 (1/2 - s/24) e,   s = hd^2 |B|^2.
+// The position stages use v only (the reference's form: q_s = h (v + sum_j a_sj q_j)),
+// so every component is v times one polynomial in h, formed once.
+// Equal to the staged FP64 step in exact arithmetic; in FP32 to its rounding.
 __device__ __forceinline__ void rk_step_mixed(double h_d, double d_d, v3<float> B, v3<double> &vel,
                                               v3<double> &pos) {
     const float h = (float)h_d, hd = (float)(h_d * d_d);
     const float vx = (float)vel.x, vy = (float)vel.y, vz = (float)vel.z;
-    float kx[6], ky[6], kz[6];
-    kx[0] = hd * (vy * B.z - vz * B.y);
-    ky[0] = hd * (vz * B.x - vx * B.z);
-    kz[0] = hd * (vx * B.y - vy * B.x);
-#define COR_STAGE(S, CX, CY, CZ)                                  \
-    {                                                             \
-        const float ux = vx + (CX), uy = vy + (CY), uz = vz + (CZ); \
-        kx[S] = hd * (uy * B.z - uz * B.y);                       \
-        ky[S] = hd * (uz * B.x - ux * B.z);                       \
-        kz[S] = hd * (ux * B.y - uy * B.x);                       \
-    }
+    const float cx = vy * B.z - vz * B.y, cy = vz * B.x - vx * B.z, cz = vx * B.y - vy * B.x;
+    const float ex = cy * B.z - cz * B.y, ey = cz * B.x - cx * B.z, ez = cx * B.y - cy * B.x;
+    const float s = hd * hd * (B.x * B.x + B.y * B.y + B.z * B.z);
+    const float al = hd * (1.f + s * (-1.f / 6.f + s * (1.f / 120.f)));
+    const float be = hd * hd * (0.5f + s * (-1.f / 24.f));
+    // position polynomial: q_s / v for the stage recurrence of src/trajectory.c's step
     const float r3 = 1.f / 3.f, r25 = 1.f / 25.f, r4 = 0.25f, r81 = 1.f / 81.f, r75 = 1.f / 75.f;
-    COR_STAGE(1, kx[0] * r3, ky[0] * r3, kz[0] * r3)
-    COR_STAGE(2, (4.f * kx[0] + 6.f * kx[1]) * r25, (4.f * ky[0] + 6.f * ky[1]) * r25,
-              (4.f * kz[0] + 6.f * kz[1]) * r25)
-    COR_STAGE(3, (kx[0] - 12.f * kx[1] + 15.f * kx[2]) * r4, (ky[0] - 12.f * ky[1] + 15.f * ky[2]) * r4,
-              (kz[0] - 12.f * kz[1] + 15.f * kz[2]) * r4)
-    COR_STAGE(4, (6.f * kx[0] + 90.f * kx[1] - 50.f * kx[2] + 8.f * kx[3]) * r81,
-              (6.f * ky[0] + 90.f * ky[1] - 50.f * ky[2] + 8.f * ky[3]) * r81,
-              (6.f * kz[0] + 90.f * kz[1] - 50.f * kz[2] + 8.f * kz[3]) * r81)
-    COR_STAGE(5, (6.f * kx[0] + 36.f * kx[1] + 10.f * kx[2] + 8.f * kx[3]) * r75,
-              (6.f * ky[0] + 36.f * ky[1] + 10.f * ky[2] + 8.f * ky[3]) * r75,
-              (6.f * kz[0] + 36.f * kz[1] + 10.f * kz[2] + 8.f * kz[3]) * r75)
-#undef COR_STAGE
-    const float r192 = 1.f / 192.f;
-    float dp[3];
-    const float vv[3] = {vx, vy, vz};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float v0 = vv[c];
-        const float q0 = h * v0;
-        const float q1 = h * (v0 + q0 * r3);
-        const float q2 = h * (v0 + (4.f * q0 + 6.f * q1) * r25);
-        const float q3 = h * (v0 + (q0 - 12.f * q1 + 15.f * q2) * r4);
-        const float q4 = h * (v0 + (6.f * q0 + 90.f * q1 - 50.f * q2 + 8.f * q3) * r81);
-        const float q5 = h * (v0 + (6.f * q0 + 36.f * q1 + 10.f * q2 + 8.f * q3) * r75);
-        dp[c] = (23.f * q0 + 125.f * q2 - 81.f * q4 + 125.f * q5) * r192;
-    }
-    vel.x += (double)((23.f * kx[0] + 125.f * kx[2] - 81.f * kx[4] + 125.f * kx[5]) * r192);
-    vel.y += (double)((23.f * ky[0] + 125.f * ky[2] - 81.f * ky[4] + 125.f * ky[5]) * r192);
-    vel.z += (double)((23.f * kz[0] + 125.f * kz[2] - 81.f * kz[4] + 125.f * kz[5]) * r192);
-    pos.x += (double)dp[0];
-    pos.y += (double)dp[1];
-    pos.z += (double)dp[2];
+    const float q0 = h;
+    const float q1 = h * (1.f + q0 * r3);
+    const float q2 = h * (1.f + (4.f * q0 + 6.f * q1) * r25);
+    const float q3 = h * (1.f + (q0 - 12.f * q1 + 15.f * q2) * r4);
+    const float q4 = h * (1.f + (6.f * q0 + 90.f * q1 - 50.f * q2 + 8.f * q3) * r81);
+    const float q5 = h * (1.f + (6.f * q0 + 36.f * q1 + 10.f * q2 + 8.f * q3) * r75);
+    const float qp = (23.f * q0 + 125.f * q2 - 81.f * q4 + 125.f * q5) * (1.f / 192.f);
+    vel.x += (double)(al * cx + be * ex);
+    vel.y += (double)(al * cy + be * ey);
+    vel.z += (double)(al * cz + be * ez);
+    pos.x += (double)(vx * qp);
+    pos.y += (double)(vy * qp);
+    pos.z += (double)(vz * qp);
 }
 
 // ---------------------------------------------------------------- helpers
@@ -468,14 +451,20 @@ __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v
     const double kt = (2. * K_PI) / (fabs(zn) * K_Q);   // period = kt * hm / b
     const double kd = 1.E-09 * zn * K_Q;                 // d = kd / hm
     const double inv_b = 1.0 / b;
+    double inv_nk0 = 1.0 / nk0;                          // halved exactly when nk0 doubles
     for (;;) {
         step++;
         const double vc = v * inv_c;
         if (vc > 1.) return ST_FORBIDDEN_REENTRY;
-        const double inv_hm = sqrt(1. - vc * vc) * inv_mass;
-        const double hm = 1.0 / inv_hm;
+        // 1/sqrt(1 - vc^2) by the hardware estimate and one Newton step (full FP64
+        // precision), in place of a square root and a division
+        const double g2 = 1. - vc * vc;
+        double rg = __builtin_amdgcn_rsq(g2);
+        rg = rg * (1.5 - 0.5 * g2 * rg * rg);
+        const double inv_hm = g2 * rg * inv_mass;
+        const double hm = a.mass * rg;
         const double period = kt * hm * inv_b;
-        const double h = period / nk0;
+        const double h = period * inv_nk0;
         const double d = kd * inv_hm;
         const v3<double> pos0 = pos, vel0 = vel;
         const double vprev = v;
@@ -490,6 +479,7 @@ __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v
         const double dot = vel.x * vel0.x + vel.y * vel0.y + vel.z * vel0.z;
         if (dot <= a.cos_tu * (v * vprev) && nk0 <= 50000) {   // acos(gc) >= tu_angle
             nk0 = (int)((unsigned)nk0 * 2u);
+            inv_nk0 *= 0.5;
             pos = pos0;
             vel = vel0;
             step--;
@@ -507,6 +497,7 @@ __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v
                 return ST_ALLOWED;
             }
             nk0 = (int)((unsigned)nk0 * 2u);
+            inv_nk0 *= 0.5;
             pos = pos0;
             vel = vel0;
             step--;
