@@ -422,15 +422,19 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         const F r2 = xx * xx + yy * yy + zz * zz;
         const F r = Dm<F>::sqrt(r2);
         const F r3 = r * r2;
-        F pas = F(0.0);
-        if (xx != F(0.0) || yy != F(0.0)) pas = Dm<F>::atan2(yy, xx);
-        const F tas = Dm<F>::atan2(Dm<F>::sqrt(xx * xx + yy * yy), zz);
-        const F stas = Dm<F>::sin(tas);
+        // the angles pas, tas by their sines and cosines: sin(tas) = rho/r,
+        // cos/sin(pas) = x/rho, y/rho, sin^2(pas/2) = (1 - cos pas)/2, tas > pi/2 for z < 0
+        const F rho = Dm<F>::sqrt(xx * xx + yy * yy);
+        const F stas = rho / r;
+        F cps = F(1.0), sps = F(0.0);
+        if (rho > F(0.0)) {
+            cps = xx / rho;
+            sps = yy / rho;
+        }
         F tet0 = Dm<F>::asin(stas / Dm<F>::pow((stas * stas * stas * stas * stas * stas) * (F(1.0) - r3) + r3,
                                                  F(0.1666666667)));
-        if (tas > F(1.5707963)) tet0 = F(3.141592654) - tet0;
-        const F sp2 = Dm<F>::sin(pas * F(0.5));
-        const F dtet = F(8.0 * 0.01745329) * (sp2 * sp2);
+        if (zz < F(0.0)) tet0 = F(3.141592654) - tet0;
+        const F dtet = F(8.0 * 0.01745329) * (F(0.5) * (F(1.0) - cps));
         const F tetr1n = tnoonn + dtet, tetr1s = tnoons - dtet;
         const F t01n = tetr1n - F(0.034906), t01s = tetr1s - F(0.034906);
         const F t02s = tetr1s + F(0.034906), t02n = tetr1n + F(0.034906);
@@ -451,7 +455,6 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
                 ct1 = -ct1;
                 ct2 = -ct2;
             }
-            const F sps = Dm<F>::sin(pas), cps = Dm<F>::cos(pas);
             const F px1 = r * st1 * cps, py1 = r * st1 * sps, pz1 = r * ct1;
             const F px2 = r * st2 * cps, py2 = r * st2 * sps, pz2 = r * ct2;
             const F ddx = px2 - px1, ddy = py2 - py1, ddz = pz2 - pz1;

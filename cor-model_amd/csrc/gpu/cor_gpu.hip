@@ -557,6 +557,7 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.tu_angle = p->tu_angle;
     a.cos_tu = p->cos_tu;
     a.mass = p->mass;
+    a.inv_mass = 1.0 / p->mass;
     a.out = p->out;
     a.coop_lanes = p->coop_lanes;
     a.run_id = ++p->run_id;      // park records of earlier runs never match

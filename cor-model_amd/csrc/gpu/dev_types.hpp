@@ -148,6 +148,7 @@ struct TraceArgs {
     double tu_angle;
     double cos_tu;
     double mass;
+    double inv_mass;             // 1 / mass
     TraceOut out;
     // cooperative tail (mixed precision): once the queue is drained, a wavefront with
     // at most coop_lanes live trajectories parks them and turns into a worker that runs

@@ -658,14 +658,31 @@ __device__ __forceinline__ v3<T> birk1tot(T x, T y, T z) {
     const T r2 = x * x + y * y + z * z;
     const T r = Dm<T>::sqrt(r2);
     const T r3 = r * r2;
-    T pas = T(0.0);
-    if (x != T(0.0) || y != T(0.0)) pas = Dm<T>::atan2(y, x);
-    const T tas = Dm<T>::atan2(Dm<T>::sqrt(x * x + y * y), z);
-    const T stas = Dm<T>::sin(tas);
+    // FP64: the reference's angles.  FP32: the same quantities without the angles --
+    // sin(tas) = rho/r, cos/sin(pas) = x/rho, y/rho, sin^2(pas/2) = (1 - cos pas)/2,
+    // tas > pi/2 where z < 0
+    T pas = T(0.0), stas, sp2s, cps = T(1.0), sps = T(0.0);
+    bool south_hemi;
+    if constexpr (Dm<T>::fp64) {
+        if (x != T(0.0) || y != T(0.0)) pas = Dm<T>::atan2(y, x);
+        const T tas = Dm<T>::atan2(Dm<T>::sqrt(x * x + y * y), z);
+        stas = Dm<T>::sin(tas);
+        south_hemi = tas > T(1.5707963);
+        const T sp2 = Dm<T>::sin(pas * T(0.5));
+        sp2s = sp2 * sp2;
+    } else {
+        const T rho = Dm<T>::sqrt(x * x + y * y);
+        stas = rho / r;
+        if (rho > T(0.0)) {
+            cps = x / rho;
+            sps = y / rho;
+        }
+        south_hemi = z < T(0.0);
+        sp2s = T(0.5) * (T(1.0) - cps);
+    }
     T tet0 = Dm<T>::asin(stas / Dm<T>::pow((stas * stas * stas * stas * stas * stas) * (T(1.0) - r3) + r3, T(0.1666666667)));
-    if (tas > T(1.5707963)) tet0 = T(3.141592654) - tet0;
-    const T sp2 = Dm<T>::sin(pas * T(0.5));
-    const T dtet = T(8.0 * 0.01745329) * (sp2 * sp2);
+    if (south_hemi) tet0 = T(3.141592654) - tet0;
+    const T dtet = T(8.0 * 0.01745329) * sp2s;
     const T tetr1n = tnoonn + dtet, tetr1s = tnoons - dtet;
     const T t01n = tetr1n - T(0.034906), t01s = tetr1s - T(0.034906);
     const T t02s = tetr1s + T(0.034906), t02n = tetr1n + T(0.034906);
@@ -688,9 +705,10 @@ __device__ __forceinline__ v3<T> birk1tot(T x, T y, T z) {
             ct1 = -ct1;
             ct2 = -ct2;
         }
-        T sps, cps;
-        sps = Dm<T>::sin(pas);
-        cps = Dm<T>::cos(pas);
+        if constexpr (Dm<T>::fp64) {
+            sps = Dm<T>::sin(pas);
+            cps = Dm<T>::cos(pas);
+        }
         px1 = r * st1 * cps;
         py1 = r * st1 * sps;
         pz1 = r * ct1;

@@ -439,18 +439,28 @@ __device__ __forceinline__ int attempts_fp64(const TraceArgs &a, const EV &E, v3
 // control arithmetic in FP64 but with reciprocal multiplications instead of the
 // reference's divisions, the turn test as gc <= cos(tu_angle) (acos is decreasing),
 // and the sphere tests on |x|^2 with a square root only near the escape radius.
+// 1/b of the mixed step (b = 1e-9 |B| in T): the hardware reciprocal square root of
+// |B|^2 with one Newton step (full FP64 precision).
+__device__ __forceinline__ double inv_b_mixed(v3<float> f) {
+    const double fx = f.x, fy = f.y, fz = f.z;
+    const double q = fx * fx + fy * fy + fz * fz;
+    double r = __builtin_amdgcn_rsq(q);
+    r = r * (1.5 - 0.5 * q * r * r);
+    return 1.E+09 * r;
+}
+
 template <class EV>
 __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v3<float> fld,
-                                              v3<double> &pos, v3<double> &vel, double &v, double b,
+                                              v3<double> &pos, v3<double> &vel, double &v, double inv_b,
                                               double zn, int &nk0, int nk1, unsigned int &step,
                                               uint32_t &attempts, double &time, double &alength, int64_t gid,
                                               bool writer = true) {
     constexpr double inv_c = 1.0 / K_C, inv_re = 1.0 / K_RE, inv_re2 = 1.0 / (K_RE * K_RE);
     constexpr double re2 = K_RE * K_RE, r25sq = (25. * K_RE) * (25. * K_RE);
-    const double inv_mass = 1.0 / a.mass;
-    const double kt = (2. * K_PI) / (fabs(zn) * K_Q);   // period = kt * hm / b
+    const double inv_mass = a.inv_mass;
+    const double azn = fabs(zn);                         // period = kt * hm / b
+    const double kt = azn == 1. ? (2. * K_PI) / K_Q : (2. * K_PI) / (azn * K_Q);
     const double kd = 1.E-09 * zn * K_Q;                 // d = kd / hm
-    const double inv_b = 1.0 / b;
     double inv_nk0 = 1.0 / nk0;                          // halved exactly when nk0 doubles
     for (;;) {
         step++;
@@ -786,9 +796,7 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
         }
         if (has) {
             evals++;
-            const double fx = f.x, fy = f.y, fz = f.z;
-            const double b = 1.E-09 * sqrt(fx * fx + fy * fy + fz * fz);
-            const int status = attempts_mixed(a, E, f, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength,
+            const int status = attempts_mixed(a, E, f, pos, vel, v, inv_b_mixed(f), zn, nk0, nk1, step, attempts, time, alength,
                                               gid, is_lead);
             if (status >= 0) {
                 if (is_lead) {
@@ -942,8 +950,7 @@ trace_kernel(TraceArgs a) {
             } else {
                 const v3<float> f = field_mixed<INT, EXT, CRU>(E, pos, gsm);
                 fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
-                const double fx = f.x, fy = f.y, fz = f.z;
-                b = 1.E-09 * sqrt(fx * fx + fy * fy + fz * fz);
+                b = inv_b_mixed(f);   // the mixed step takes 1/b
             }
 
             // ---- 3. attempts until one is accepted or the trajectory ends
