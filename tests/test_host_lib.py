@@ -149,6 +149,20 @@ def test_gpu_calls_fail_cleanly_without_a_gpu():
     assert rc == A.COR_E_NO_DEVICE
 
 
+def test_field_precision_checked_first():
+    """cor_gpu_field rejects a precision outside FP64, mixed and the four cooperative
+    evaluators (64, 8, 16, 32 lanes) before any device work."""
+    lib = A.lib()
+    p = A.recalc(2020)
+    out = np.zeros(3)
+    pts = np.zeros(3)
+    for bad in (-1, 6, 99):
+        rc = lib.cor_gpu_field(C.byref(A.model_set()), C.byref(p), pts.ctypes.data_as(A.DP),
+                               out.ctypes.data_as(A.DP), 1, bad)
+        assert rc == A.COR_E_INVALID, bad
+    assert (A.PREC_MIXED_GROUP, A.PREC_MIXED_GROUP16, A.PREC_MIXED_GROUP32) == (3, 4, 5)
+
+
 def test_epoch_series_argument_checks():
     """Invalid arguments of the epoch-series entry points are rejected before any
     device work (the same checks run on a GPU host)."""
