@@ -57,6 +57,9 @@ __device__ __forceinline__ void pin(v3<T> &v) {
 #define COR_CTAB(name, n, ...)                                      \
     static __constant__ double name##_d[n] = {__VA_ARGS__};        \
     static __constant__ float name##_f[n] = {__VA_ARGS__};
+// FP32-only tables of derived constants (t96_derived.h, written by gen_t96_derived.py)
+#define COR_FTAB(name, n, ...) static __constant__ float name##_f[n] = {__VA_ARGS__};
+#include "t96_derived.h"
 __device__ __forceinline__ const double *tab_sel(const double *d, const float *, double) { return d; }
 __device__ __forceinline__ const float *tab_sel(const double *, const float *f, float) { return f; }
 #define CT(name) tab_sel(name##_d, name##_f, T(0))
@@ -394,6 +397,29 @@ __device__ __forceinline__ void tail87(T x, T z, T *bx, T *bz) {
 template <class T>
 __device__ __forceinline__ v3<T> shlcar3x3(int set, T x, T y, T z) {
     const T *tab = CT(k_shl) + 15 * set;
+    if constexpr (!Dm<T>::fp64) {
+        // FP32: reciprocal scales and decay rates from t96_derived.h
+        const float *rpt = k_shl_rp_f + 3 * set, *rkt = k_shl_rk_f + 3 * set, *sqt = k_shl_sq_f + 9 * set;
+        T szr[3], czr[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) Dm<T>::sincos(z * rkt[k], &szr[k], &czr[k]);
+        v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            T sy, cy;
+            Dm<T>::sincos(y * rpt[i], &sy, &cy);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const T sq = sqt[3 * i + k];
+                const T e = Dm<T>::exp(x * sq);
+                const T c = tab[3 * i + k];
+                b.x += (-sq * e * cy * szr[k]) * c;
+                b.y += (e * rpt[i] * sy * szr[k]) * c;
+                b.z += (-e * rkt[k] * cy * czr[k]) * c;
+            }
+        }
+        return b;
+    }
     T szr[3], czr[3], rinv2[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -425,6 +451,37 @@ __device__ __forceinline__ v3<T> shlcar3x3(int set, T x, T y, T z) {
 // lib/t96/t96.c:854-881
 template <class T>
 __device__ __forceinline__ v3<T> circle(T x, T y, T z, T rl) {
+    if constexpr (!Dm<T>::fp64) {
+        // FP32: the same expressions with a reciprocal square root for 1/r2 and one
+        // reciprocal per divisor (five transcendental instructions instead of eleven)
+        const T rho2 = x * x + y * y;
+        const T rho = Dm<T>::sqrt(rho2);
+        const T rr = rho + rl;
+        const T r22 = z * z + rr * rr;
+        const T ir2 = __builtin_amdgcn_rsqf(r22);
+        const T r12 = r22 - T(4.0) * rho * rl;
+        const T ir12 = __builtin_amdgcn_rcpf(r12);
+        const T r32 = T(0.5) * (r12 + r22);
+        const T xk2s = r12 * (ir2 * ir2);   // r12 / r22 without the 1 - (1 - q) cancellation
+        const T dl = -Dm<T>::log(xk2s);
+        const T kk = T(1.38629436112) + xk2s * (T(0.09666344259) + xk2s * (T(0.03590092383) +
+                     xk2s * (T(0.03742563713) + xk2s * T(0.01451196212)))) + dl *
+                     (T(0.5) + xk2s * (T(0.12498593597) + xk2s * (T(0.06880248576) +
+                     xk2s * (T(0.03328355346) + xk2s * T(0.00441787012)))));
+        const T ee = T(1.0) + xk2s * (T(0.44325141463) + xk2s * (T(0.0626060122) + xk2s *
+                     (T(0.04757383546) + xk2s * T(0.01736506451)))) + dl *
+                     xk2s * (T(0.2499836831) + xk2s * (T(0.09200180037) + xk2s *
+                     (T(0.04069697526) + xk2s * T(0.00526449639))));
+        T brho;
+        if (rho > T(1e-6)) {
+            brho = z * __builtin_amdgcn_rcpf(rho2) * ir2 * (r32 * ir12 * ee - kk);
+        } else {
+            brho = T(3.141592654) * rl * ir2 * (rl - rho) * ir12 * z / (r32 - rho2);
+        }
+        v3<T> out = mk3<T>(brho * x, brho * y, (kk - ee * (r32 - T(2.0) * rl * rl) * ir12) * ir2);
+        pin(out);
+        return out;
+    }
     const T rho2 = x * x + y * y;
     const T rho = Dm<T>::sqrt(rho2);
     const T rr = rho + rl;
@@ -619,6 +676,29 @@ __device__ __forceinline__ v3<T> condip_sum(T x, T y, T z) {
 template <class T>
 __device__ __forceinline__ v3<T> birk1shld(T x, T y, T z) {
     const T *py = CT(k_b1s_py), *pz = CT(k_b1s_pz), *amp = CT(k_b1s_amp);
+    if constexpr (!Dm<T>::fp64) {
+        // FP32: reciprocal scales and decay rates from t96_derived.h
+        T szk[4], czk[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) Dm<T>::sincos(z * k_b1s_rk_f[k], &szk[k], &czk[k]);
+        v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const T rp = k_b1s_rp_f[i];
+            T sy, cy;
+            Dm<T>::sincos(rp * y, &sy, &cy);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const T sq = k_b1s_sq_f[4 * i + k];
+                const T e = Dm<T>::exp(x * sq);
+                const T c = amp[4 * i + k];
+                b.x += (-sq * e * cy * szk[k]) * c;
+                b.y += (rp * e * sy * szk[k]) * c;
+                b.z += (-k_b1s_rk_f[k] * e * cy * czk[k]) * c;
+            }
+        }
+        return b;
+    }
     T rk[4], rk2[4], szk[4], czk[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -762,6 +842,28 @@ __device__ __forceinline__ v3<T> birk2shl(T x, T y, T z) {
     const T ak[2] = {T(10.21914434), T(10.09021632)};
     const double amp[4] = {-111.6371348 + (double)124.5402702, 110.3735178 + (double)-122.0095905,
                            111.9448247 + (double)-129.1957743, -110.7586562 + (double)126.5649012};
+    if constexpr (!Dm<T>::fp64) {
+        // FP32: reciprocal scales and decay rates from t96_derived.h
+        T szk[2], czk[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) Dm<T>::sincos(z * k_b2s_rk_f[k], &szk[k], &czk[k]);
+        v3<T> b = mk3<T>(0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            T sy, cy;
+            Dm<T>::sincos(y * k_b2s_rp_f[i], &sy, &cy);
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const T sq = k_b2s_sq_f[2 * i + k];
+                const T e = Dm<T>::exp(x * sq);
+                const T c = T(amp[2 * i + k]);
+                b.x += (-sq * e * cy * szk[k]) * c;
+                b.y += (e * k_b2s_rp_f[i] * sy * szk[k]) * c;
+                b.z += (-e * k_b2s_rk_f[k] * cy * czk[k]) * c;
+            }
+        }
+        return b;
+    }
     T szk[2], czk[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) Dm<T>::sincos(z / ak[k], &szk[k], &czk[k]);

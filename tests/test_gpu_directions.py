@@ -7,8 +7,9 @@ allowed rigidity (src/trajectory.c:91-112); a finer scan over the penumbra is an
 run with a smaller del.  Directions given as local zenith/azimuth are converted with
 cor_direction_to_geo (FP64, host); refined scans come from cor_refine_scan.  Bars as
 in test_gpu_trace: FP64 flags equal outside the union penumbra band, |dRc| within one
-coarse step, asymptotic directions within 0.05 degree; mixed per-lane within 0.05 GV
-and 0.5 degree.
+coarse step, asymptotic directions within 0.05 degree; mixed per-lane: flags equal
+outside the union band, |dRc| within half that band (at least one coarse step),
+directions within 0.5 degree.
 """
 import numpy as np
 import pytest
@@ -63,7 +64,11 @@ def test_oblique_directions_against_oracle(cor, oracle, precision):
         if precision == 0:
             compare(row, tg[off:off + n], cg[k], tc, cc, rc_tol=0.25)
         else:
-            compare(row, tg[off:off + n], cg[k], tc, cc, rc_tol=max(0.25, 0.5 * (cc[1] - cc[0])), dir_tol=0.5,
+            # half the union penumbra band of the two runs (at least one coarse step):
+            # inside the band single trajectories are chaotic, and a last-bit change of
+            # the FP32 field moves the lowest allowed rigidity by whole steps
+            band = max(cg[k]["ru"], cc[1]) - min(cg[k]["rl"], cc[0])
+            compare(row, tg[off:off + n], cg[k], tc, cc, rc_tol=max(0.25, 0.5 * band), dir_tol=0.5,
                     dir_margin=1.0, exit_tol=5.0)
         off += n
     # oblique directions see other cutoffs than the vertical one at the same station

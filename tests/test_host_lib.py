@@ -233,3 +233,19 @@ def test_chaos716_table_matches_the_reference():
     want = cv["chaos716_b"]
     same = (got == want) | (np.isnan(got) & np.isnan(want))
     assert same.all(), np.argwhere(~same)[:5]
+
+
+def test_t96_derived_tables_match_generator():
+    """t96_derived.h (FP32 reciprocal scales and decay rates of the T96 shield
+    harmonics) is what gen_t96_derived.py computes from t96_dev.hpp's coefficient
+    tables; and a decay rate is sqrt(1/p^2 + 1/r^2) of the ring-current set's first
+    y and z scales (12.75434981, 1.752483754, lib/t96/t96.c:274-299)."""
+    import importlib.util
+    gpu = Path(__file__).resolve().parents[1] / "cor-model_amd" / "csrc" / "gpu"
+    spec = importlib.util.spec_from_file_location("gen_t96_derived", gpu / "gen_t96_derived.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    assert gen.render() == (gpu / "t96_derived.h").read_text()
+    tabs = dict(gen.derived())
+    assert abs(tabs["k_shl_sq"][0] - (1 / 12.75434981 ** 2 + 1 / 1.752483754 ** 2) ** 0.5) < 1e-15
+    assert abs(tabs["k_shl_rp"][0] * 12.75434981 - 1) < 1e-15 and abs(tabs["k_shl_rk"][0] * 1.752483754 - 1) < 1e-15
