@@ -820,9 +820,13 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
             F bphi = F(0.);
             F q = pmm, dq = dmm, q2 = F(0.), dq2 = F(0.), an = am;
             int idx = m * (m - 1) / 2 + m - 1;
+            // coefficients two degrees ahead are read while this degree is summed (LDS
+            // latency; the tables are zero-padded past the last index read)
             F g = G[idx], h = H[idx], rec = REC[idx];
+            F gn = G[idx + m], hn = H[idx + m], recn = REC[idx + m];
             for (int n = m; n <= kw; n++) {
-                const F gn = G[idx + n], hn = H[idx + n], recn = REC[idx + n];   // tables padded to 128
+                const int i2 = idx + n + (n + 1);
+                const F g2 = G[i2], h2 = H[i2], rec2 = REC[i2];
                 const F w = g * sm + h * cm;
                 br = br + (an * n) * w * q;
                 bt = bt - an * w * dq;
@@ -838,6 +842,9 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
                 g = gn;
                 h = hn;
                 rec = recn;
+                gn = g2;
+                hn = h2;
+                recn = rec2;
             }
             if (m >= 2) bf = bf + bphi * (F)(m - 1);
         }

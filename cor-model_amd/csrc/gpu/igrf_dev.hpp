@@ -18,9 +18,10 @@
 
 namespace cor {
 
-// Coefficient tables as staged in LDS: 105 entries padded so the one-ahead prefetch of
-// the last degree stays inside the array.
-constexpr int IGRF_LDS_N = 128;
+// Coefficient tables as staged in LDS: 105 entries zero-padded so the prefetches of the
+// last degrees (one degree ahead here, two in the cooperative evaluator: index at most
+// 133) stay inside the array.
+constexpr int IGRF_LDS_N = 136;
 
 template <class T>
 __device__ __forceinline__ v3<T> igrf_geo(const T *__restrict__ G, const T *__restrict__ H,

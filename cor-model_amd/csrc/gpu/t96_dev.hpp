@@ -472,12 +472,12 @@ __device__ __forceinline__ v3<T> circle(T x, T y, T z, T rl) {
                      (T(0.04757383546) + xk2s * T(0.01736506451)))) + dl *
                      xk2s * (T(0.2499836831) + xk2s * (T(0.09200180037) + xk2s *
                      (T(0.04069697526) + xk2s * T(0.00526449639))));
-        T brho;
-        if (rho > T(1e-6)) {
-            brho = z * __builtin_amdgcn_rcpf(rho2) * ir2 * (r32 * ir12 * ee - kk);
-        } else {
-            brho = T(3.141592654) * rl * ir2 * (rl - rho) * ir12 * z / (r32 - rho2);
-        }
+        // near the axis (rho <= 1e-6) the reference's limit form; one reciprocal serves
+        // whichever form the lane takes
+        const bool off_axis = rho > T(1e-6);
+        const T iden = __builtin_amdgcn_rcpf(off_axis ? rho2 : r32 - rho2);
+        const T brho = off_axis ? z * iden * ir2 * (r32 * ir12 * ee - kk)
+                                : T(3.141592654) * rl * ir2 * (rl - rho) * ir12 * z * iden;
         v3<T> out = mk3<T>(brho * x, brho * y, (kk - ee * (r32 - T(2.0) * rl * rl) * ir12) * ir2);
         pin(out);
         return out;
