@@ -159,7 +159,7 @@ def main():
     ap.add_argument("--handover-steps", type=int, default=0,
                     help="per-lane steps after which a trajectory is handed to the cooperative mode "
                          "on its own once the queue is drained (0: library default, negative: off)")
-    ap.add_argument("--fp64-stride", type=int, default=16,
+    ap.add_argument("--fp64-stride", type=int, default=32,
                     help="FP64 comparison run on every fp64-stride-th site of the benchmarked sites")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
