@@ -539,6 +539,15 @@ __device__ __forceinline__ int load_acquire(int32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 // relaxed load of a shared counter, made wavefront-uniform
+// A lane's double read by every lane, the lane index being wavefront-uniform (two
+// readlanes through scalar registers instead of an LDS permute).
+__device__ __forceinline__ double read_lane_d(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, src);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ unsigned long long load_relaxed_uniform(unsigned long long *p) {
     const unsigned long long x = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)x);
@@ -782,8 +791,8 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
         v3<float> f;
         const int k_live = __popcll(has_leads);
         if (k_live == 1) {
-            const int src = __ffsll((unsigned long long)has_leads) - 1;
-            const v3<double> p1 = mk3(__shfl(pos.x, src), __shfl(pos.y, src), __shfl(pos.z, src));
+            const int src = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)has_leads) - 1);
+            const v3<double> p1 = mk3(read_lane_d(pos.x, src), read_lane_d(pos.y, src), read_lane_d(pos.z, src));
             const v3<double> gsm = rot_geo_to_gsm(E.A, p1);
             f = coop_field<INT, EXT, 64>(E.G, E.H, E.REC, E.Af, E.t96, T, p1, gsm, lane);
         } else if (k_live == 2) {
