@@ -674,13 +674,18 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         if (w != F(0.0)) {
             const int m = (int)c.x;
             const F X = (g0 ? cxp : xx) + c.z, Y = g0 ? cyp : yy, Z = g0 ? czp : zz;
+            // reciprocal square roots in place of the roots and divisions:
+            // 1/(r s) = 1/ro, 1/ch^2 = 1/(0.5 (1 + cos)), 1/sh^2 = 1/(0.5 (1 - cos))
             const F ro2 = X * X + Y * Y;
-            const F ro = Dm<F>::sqrt(ro2);
-            const F cf = X / ro, sf = Y / ro;
-            const F r = Dm<F>::sqrt(ro2 + Z * Z);
-            const F cc = Z / r, s = ro / r;
-            const F ch = Dm<F>::sqrt(F(0.5) * (F(1.0) + cc)), sh = Dm<F>::sqrt(F(0.5) * (F(1.0) - cc));
-            const F tnh = sh / ch, cnh = F(1.0) / tnh;
+            const F irs = __builtin_amdgcn_rsqf(ro2);
+            const F ro = ro2 * irs;
+            const F cf = X * irs, sf = Y * irs;
+            const F rr2 = ro2 + Z * Z;
+            const F ir = __builtin_amdgcn_rsqf(rr2);
+            const F cc = Z * ir, s = ro * ir;
+            const F ch2 = F(0.5) * (F(1.0) + cc), sh2 = F(0.5) * (F(1.0) - cc);
+            const F ich = __builtin_amdgcn_rsqf(ch2), ish = __builtin_amdgcn_rsqf(sh2);
+            const F tnh = sh2 * ish * ich, cnh = ch2 * ich * ish;
             // cos/sin(m phi), tnh^m, cnh^m and the (m-1)-th powers
             F cfm = cf, sfm = sf, tm = tnh, cm = cnh, tm1 = F(1.0), cm1 = F(1.0);
 #pragma unroll
@@ -696,8 +701,8 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
                 }
             }
             const F fm = (F)m;
-            const F bt = fm * cfm / (r * s) * (tm + cm);
-            const F bf = F(-0.5) * fm * sfm / r * (tm1 / (ch * ch) - cm1 / (sh * sh));
+            const F bt = fm * cfm * irs * (tm + cm);
+            const F bf = F(-0.5) * fm * sfm * ir * (tm1 * (ich * ich) - cm1 * (ish * ish));
             const F k = w * c.y;
             px += k * (bt * cc * cf - bf * sf);
             py += k * (bt * cc * sf + bf * cf);

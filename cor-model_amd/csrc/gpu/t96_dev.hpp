@@ -604,13 +604,39 @@ __device__ __forceinline__ v3<T> condip_sum(T x, T y, T z) {
     }
     const T xsm = x + T(0.16);
     const T ro2 = xsm * xsm + y * y;
-    const T ro = Dm<T>::sqrt(ro2);
-    const T cf0 = xsm / ro, sf0 = y / ro;
     const T r2 = ro2 + z * z;
-    const T r = Dm<T>::sqrt(r2);
-    const T c = z / r, s = ro / r;
-    const T ch = Dm<T>::sqrt(T(0.5) * (T(1.0) + c)), sh = Dm<T>::sqrt(T(0.5) * (T(1.0) - c));
-    const T tnh = sh / ch, cnh = T(1.0) / tnh;
+    // FP32: reciprocal square roots in place of the roots and divisions
+    // (1/(r s) = 1/ro, 1/ch^2 = 1/(0.5 (1 + c)), 1/sh^2 = 1/(0.5 (1 - c)))
+    T ro, cf0, sf0, r, c, s, ch, sh, tnh, cnh, irs = T(0), ir = T(0), ich2 = T(0), ish2 = T(0);
+    if constexpr (Dm<T>::fp64) {
+        ro = Dm<T>::sqrt(ro2);
+        cf0 = xsm / ro;
+        sf0 = y / ro;
+        r = Dm<T>::sqrt(r2);
+        c = z / r;
+        s = ro / r;
+        ch = Dm<T>::sqrt(T(0.5) * (T(1.0) + c));
+        sh = Dm<T>::sqrt(T(0.5) * (T(1.0) - c));
+        tnh = sh / ch;
+        cnh = T(1.0) / tnh;
+    } else {
+        irs = __builtin_amdgcn_rsqf(ro2);
+        ro = ro2 * irs;
+        cf0 = xsm * irs;
+        sf0 = y * irs;
+        ir = __builtin_amdgcn_rsqf(r2);
+        r = r2 * ir;
+        c = z * ir;
+        s = ro * ir;
+        const T ch2 = T(0.5) * (T(1.0) + c), sh2 = T(0.5) * (T(1.0) - c);
+        const T ich = __builtin_amdgcn_rsqf(ch2), ish = __builtin_amdgcn_rsqf(sh2);
+        ch = ch2 * ich;
+        sh = sh2 * ish;
+        tnh = sh * ich;
+        cnh = ch * ish;
+        ich2 = ich * ich;
+        ish2 = ish * ish;
+    }
     // conical harmonics m = 1..5 with cos/sin(m phi) by the reference's recurrence
     T cfm = cf0, sfm = sf0;
     T tp = T(1.0), cp = T(1.0);     // tnh^m, cnh^m (FP32 path; the FP64 path calls pow)
@@ -639,8 +665,14 @@ __device__ __forceinline__ v3<T> condip_sum(T x, T y, T z) {
             tm1 = tp;
             cm1 = cp;
         }
-        const T bt = m1 * cfm / (r * s) * (tm1 + cm1);
-        const T bf = T(-0.5) * m1 * sfm / r * (tm / (ch * ch) - cmm / (sh * sh));
+        T bt, bf;
+        if constexpr (Dm<T>::fp64) {
+            bt = m1 * cfm / (r * s) * (tm1 + cm1);
+            bf = T(-0.5) * m1 * sfm / r * (tm / (ch * ch) - cmm / (sh * sh));
+        } else {
+            bt = m1 * cfm * irs * (tm1 + cm1);
+            bf = T(-0.5) * m1 * sfm * ir * (tm * ich2 - cmm * ish2);
+        }
         COR_ACC(m, bt * c * cf0 - bf * sf0, bt * c * sf0 + bf * cf0, -bt * s)
     }
     for (int i = 0; i < 9; i++) {
@@ -998,14 +1030,39 @@ template <class T>
 __device__ __forceinline__ v3<T> r2inner(T x, T y, T z) {
     // bconic (lib/t96/t96.c:1338-1366)
     const T ro2 = x * x + y * y;
-    const T ro = Dm<T>::sqrt(ro2);
-    const T cf = x / ro, sf = y / ro;
     const T r2 = ro2 + z * z;
-    const T r = Dm<T>::sqrt(r2);
-    const T c = z / r, s = ro / r;
-    const T ch = Dm<T>::sqrt(T(0.5) * (T(1.0) + c)), sh = Dm<T>::sqrt(T(0.5) * (T(1.0) - c));
+    // FP32: reciprocal square roots in place of the roots and divisions (as in condip_sum)
+    T ro, cf, sf, r, c, s, ch, sh, tnh, cnh, irs = T(0), ir = T(0), ich2 = T(0), ish2 = T(0);
+    if constexpr (Dm<T>::fp64) {
+        ro = Dm<T>::sqrt(ro2);
+        cf = x / ro;
+        sf = y / ro;
+        r = Dm<T>::sqrt(r2);
+        c = z / r;
+        s = ro / r;
+        ch = Dm<T>::sqrt(T(0.5) * (T(1.0) + c));
+        sh = Dm<T>::sqrt(T(0.5) * (T(1.0) - c));
+        tnh = sh / ch;
+        cnh = T(1.0) / tnh;
+    } else {
+        irs = __builtin_amdgcn_rsqf(ro2);
+        ro = ro2 * irs;
+        cf = x * irs;
+        sf = y * irs;
+        ir = __builtin_amdgcn_rsqf(r2);
+        r = r2 * ir;
+        c = z * ir;
+        s = ro * ir;
+        const T ch2 = T(0.5) * (T(1.0) + c), sh2 = T(0.5) * (T(1.0) - c);
+        const T ich = __builtin_amdgcn_rsqf(ch2), ish = __builtin_amdgcn_rsqf(sh2);
+        ch = ch2 * ich;
+        sh = sh2 * ish;
+        tnh = sh * ich;
+        cnh = ch * ish;
+        ich2 = ich * ich;
+        ish2 = ish * ish;
+    }
     T tnh_prev = T(1.0), cnh_prev = T(1.0);
-    const T tnh = sh / ch, cnh = T(1.0) / tnh;
     T cfm_prev = T(1.0), sfm_prev = T(0.0);
     const T cc[5] = {T(154.185), T(-2.12446), T(0.601735E-01), T(-.153954E-02), T(.355077E-04)};
     v3<T> cb[5];
@@ -1016,8 +1073,14 @@ __device__ __forceinline__ v3<T> r2inner(T x, T y, T z) {
         cfm_prev = cfm;
         sfm_prev = sfm;
         const T tnhm = tnh_prev * tnh, cnhm = cnh_prev * cnh;
-        const T bt = T(m + 1) * cfm / (r * s) * (tnhm + cnhm);
-        const T bf = T(-0.5) * T(m + 1) * sfm / r * (tnh_prev / (ch * ch) - cnh_prev / (sh * sh));
+        T bt, bf;
+        if constexpr (Dm<T>::fp64) {
+            bt = T(m + 1) * cfm / (r * s) * (tnhm + cnhm);
+            bf = T(-0.5) * T(m + 1) * sfm / r * (tnh_prev / (ch * ch) - cnh_prev / (sh * sh));
+        } else {
+            bt = T(m + 1) * cfm * irs * (tnhm + cnhm);
+            bf = T(-0.5) * T(m + 1) * sfm * ir * (tnh_prev * ich2 - cnh_prev * ish2);
+        }
         tnh_prev = tnhm;
         cnh_prev = cnhm;
         cb[m] = mk3<T>(bt * c * cf - bf * sf, bt * c * sf + bf * cf, -bt * s);
