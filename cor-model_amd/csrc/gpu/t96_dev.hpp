@@ -158,14 +158,21 @@ __device__ __forceinline__ void bessel01(T x, T *j0, T *j1) {
         const T j1x = T(0.5) - q * (T(0.56249985) - q * (T(0.21093573) - q * (T(0.03954289) - q * (T(0.00443319) - q * (T(0.00031761) - q * T(0.00001109))))));
         *j1 = j1x * x;
     } else {
-        const T u = T(3.0) / x;
-        const T sx = Dm<T>::sqrt(x);
+        // FP32: 1/sqrt(x) once for u = 3/x and the 1/sqrt(x) factors
+        T u, isx = T(0), sx = T(1);
+        if constexpr (Dm<T>::fp64) {
+            u = T(3.0) / x;
+            sx = Dm<T>::sqrt(x);
+        } else {
+            isx = __builtin_amdgcn_rsqf(x);
+            u = T(3.0) * (isx * isx);
+        }
         const T f0 = T(0.79788456) - u * (T(0.00000077) + u * (T(0.00552740) + u * (T(0.00009512) - u * (T(0.00137237) - u * (T(0.00072805) - u * T(0.00014476))))));
         const T t0 = x - T(0.78539816) - u * (T(0.04166397) + u * (T(0.00003954) - u * (T(0.00262573) - u * (T(0.00054125) + u * (T(0.00029333) - u * T(0.00013558))))));
-        *j0 = f0 / sx * Dm<T>::cos(t0);
+        *j0 = (Dm<T>::fp64 ? f0 / sx : f0 * isx) * Dm<T>::cos(t0);
         const T f1 = T(0.79788456) + u * (T(0.00000156) + u * (T(0.01659667) + u * (T(0.00017105) - u * (T(0.00249511) - u * (T(0.00113653) - u * T(0.00020033))))));
         const T t1 = x - T(2.35619449) + u * (T(0.12499612) + u * (T(0.0000565) - u * (T(0.00637879) - u * (T(0.00074348) + u * (T(0.00079824) - u * T(0.00029166))))));
-        *j1 = f1 / sx * Dm<T>::cos(t1);
+        *j1 = (Dm<T>::fp64 ? f1 / sx : f1 * isx) * Dm<T>::cos(t1);
     }
 }
 
@@ -281,6 +288,28 @@ __device__ __forceinline__ void cone_terms(T zb, T rhos, T bi, T *as, T *das1, T
                                            T *ds1dz, T *ds2dz, T *ds1dr, T *ds2dr) {
     const T rp = rhos + bi, rm = rhos - bi;
     const T zb2 = zb * zb, b2 = T(2.0) * bi;
+    if constexpr (!Dm<T>::fp64) {
+        // FP32: three reciprocal square roots and one reciprocal for the eleven roots
+        // and divisions below
+        const T q1 = zb2 + rp * rp, q2 = zb2 + rm * rm;
+        const T is1 = __builtin_amdgcn_rsqf(q1), is2 = __builtin_amdgcn_rsqf(q2);
+        const T s1 = q1 * is1, s2 = q2 * is2;
+        *ds1dz = zb * is1;
+        *ds2dz = zb * is2;
+        *ds1dr = rp * is1;
+        *ds2dr = rm * is2;
+        const T sp = s1 + s2, sp2 = sp * sp;
+        const T isp = __builtin_amdgcn_rcpf(sp), isp2 = isp * isp, is12 = is1 * is2;
+        const T q3 = sp2 - b2 * b2;
+        const T ifac1 = __builtin_amdgcn_rsqf(q3);
+        const T fac1 = q3 * ifac1;
+        *as = fac1 * is12 * isp2;
+        const T term1 = is12 * isp * ifac1;
+        const T fac2 = *as * isp2;
+        *das1 = term1 - fac2 * is1 * (s2 * s2 + s1 * (T(3.0) * s1 + T(4.0) * s2));
+        *das2 = term1 - fac2 * is2 * (s1 * s1 + s2 * (T(3.0) * s2 + T(4.0) * s1));
+        return;
+    }
     const T s1 = Dm<T>::sqrt(zb2 + rp * rp), s2 = Dm<T>::sqrt(zb2 + rm * rm);
     *ds1dz = zb / s1;
     *ds2dz = zb / s2;
@@ -534,7 +563,13 @@ template <class T>
 __device__ __forceinline__ void dipxyz(T x, T y, T z, v3<T> *mx, v3<T> *my, v3<T> *mz) {
     const T x2 = x * x, y2 = y * y, z2 = z * z;
     const T r2 = x2 + y2 + z2;
-    const T q = T(30574.0) / (r2 * r2 * Dm<T>::sqrt(r2));
+    T q;
+    if constexpr (Dm<T>::fp64) {
+        q = T(30574.0) / (r2 * r2 * Dm<T>::sqrt(r2));
+    } else {
+        const T i = __builtin_amdgcn_rsqf(r2), i2 = i * i;
+        q = T(30574.0) * (i2 * i2) * i;
+    }
     const T q3 = T(3.0) * q;
     mx->x = q * (T(3.0) * x2 - r2);
     mx->y = q3 * x * y;
