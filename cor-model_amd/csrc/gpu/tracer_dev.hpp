@@ -439,14 +439,17 @@ __device__ __forceinline__ int attempts_fp64(const TraceArgs &a, const EV &E, v3
 // control arithmetic in FP64 but with reciprocal multiplications instead of the
 // reference's divisions, the turn test as gc <= cos(tu_angle) (acos is decreasing),
 // and the sphere tests on |x|^2 with a square root only near the escape radius.
-// 1/b of the mixed step (b = 1e-9 |B| in T): the hardware reciprocal square root of
-// |B|^2 with one Newton step (full FP64 precision).
+// 1/sqrt(q) in FP64 for the mixed step: the hardware estimate and one Newton step
+// (full precision for its uses: speeds, lengths, the Lorentz factor, 1/|B|).
+__device__ __forceinline__ double rsq_fp64(double q) {
+    const double r = __builtin_amdgcn_rsq(q);
+    return r * (1.5 - 0.5 * q * r * r);
+}
+
+// 1/b of the mixed step (b = 1e-9 |B| in T).
 __device__ __forceinline__ double inv_b_mixed(v3<float> f) {
     const double fx = f.x, fy = f.y, fz = f.z;
-    const double q = fx * fx + fy * fy + fz * fz;
-    double r = __builtin_amdgcn_rsq(q);
-    r = r * (1.5 - 0.5 * q * r * r);
-    return 1.E+09 * r;
+    return 1.E+09 * rsq_fp64(fx * fx + fy * fy + fz * fz);
 }
 
 template <class EV>
@@ -469,8 +472,7 @@ __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v
         // 1/sqrt(1 - vc^2) by the hardware estimate and one Newton step (full FP64
         // precision), in place of a square root and a division
         const double g2 = 1. - vc * vc;
-        double rg = __builtin_amdgcn_rsq(g2);
-        rg = rg * (1.5 - 0.5 * g2 * rg * rg);
+        const double rg = rsq_fp64(g2);
         const double inv_hm = g2 * rg * inv_mass;
         const double hm = a.mass * rg;
         const double period = kt * hm * inv_b;
@@ -481,7 +483,7 @@ __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v
         rk_step_mixed(h, d, fld, vel, pos);
         attempts++;
         const double v2 = vel.x * vel.x + vel.y * vel.y + vel.z * vel.z;
-        v = sqrt(v2);
+        v = v2 * rsq_fp64(v2);   // NaN stays NaN (and is caught below)
         if (__builtin_isnan(v)) {
             step = (unsigned int)(a.step_limit + 1);
             return ST_FORBIDDEN_LIMIT;
@@ -517,7 +519,8 @@ __device__ __forceinline__ int attempts_mixed(const TraceArgs &a, const EV &E, v
         if ((uint64_t)step > a.step_limit) return ST_FORBIDDEN_LIMIT;
         time = time + h;
         const double dx = pos.x - pos0.x, dy = pos.y - pos0.y, dz = pos.z - pos0.z;
-        alength += sqrt(dx * dx + dy * dy + dz * dz);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        alength += d2 > 0. ? d2 * rsq_fp64(d2) : 0.;
         nk0 = nk1;
         return -1;
     }
