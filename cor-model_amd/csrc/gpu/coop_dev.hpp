@@ -1,20 +1,23 @@
-// coop_dev.hpp -- cooperative mode of the mixed tracer: one wavefront per trajectory.
+// coop_dev.hpp -- cooperative mode of the mixed tracer: a group of lanes (8, 16, 32 or
+// the whole wavefront) per trajectory.
 //
 // Cutoff scans have heavy-tailed step counts: a few trapped trajectories (lowest
 // rigidities) run 1e5..1e6 steps, hundreds of times the median.  In the per-lane mode a
 // wavefront's step takes as long with one live lane as with 64, so once the queue is
 // drained the run time is the latency of those few trajectories.  In cooperative mode
-// the 64 lanes of a wavefront share one trajectory:
-//   * the trajectory state is held by every lane (uniform values, identical arithmetic);
+// the lanes of a group share one trajectory:
+//   * the trajectory state is held by every lane of the group (uniform values,
+//     identical arithmetic);
 //   * the field model is split into families of terms of identical form -- Cartesian
 //     harmonics, dipoles, current loops, conical harmonics, conical sheets, cylindrical
 //     harmonics, the tail-87 sheets, the Region-2 sheet terms, IGRF orders -- one term
-//     per lane, so a family is a single SIMD pass whatever its size;
+//     per lane, so a family is one SIMD pass, or a few for small groups;
 //   * each lane accumulates its terms; one cross-lane sum per step (DPP, read back from
 //     one lane) gives every lane the same bits;
 //   * the RK attempts and step control are the per-lane mode's code, run redundantly.
 // The region switches of T96 (magnetopause blend, Region-1 bands, Region-2 xi bands) are
-// uniform per step, so no family pass diverges on them.
+// uniform per trajectory, so a lone trajectory's family passes never diverge on them
+// (groups of a wavefront at different points may).
 //
 // Arithmetic is FP32 like the per-lane mixed path.  The interconnection harmonics (a fit
 // whose terms cancel by four orders of magnitude) are summed in a form without the
@@ -342,8 +345,8 @@ __device__ __forceinline__ void put4(float *d, float4 v) {
 }
 
 // gl: the lane's index in its group.  Families with more terms than the group has
-// lanes (groups of 16: the Cartesian harmonics and the current loops; groups of 8 also
-// the conical harmonics and the Region-2 sheet) are read in their passes instead.
+// lanes (groups of 16 or 32: the Cartesian harmonics and the current loops; groups of 8
+// also the conical harmonics and the Region-2 sheet) are read in their passes instead.
 template <int G>
 __device__ __forceinline__ void load_lane(const CoopTab &T, int gl, CoopLane &L) {
     if constexpr (G == 64) L.harm = ld4(T.harm[gl]);

@@ -261,9 +261,10 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
 // Once the queue is drained, a wavefront with at most COOP_LANES_DEFAULT live
 // trajectories hands them to the cooperative mode, and a fuller one does so when
 // waiting workers times COOP_IDLE_WEIGHT reach its live count: one waiting worker starts
-// taking a wavefront's trajectories, the rest are claimed four to a worker from the
-// parked pool.  Both measured on the 5-degree IGRF + T96 grid (full grid: 21.8 s with
-// 4 and 1, 17.5 s with 32 and 64).
+// taking a wavefront's trajectories, the rest are claimed up to eight to a worker from
+// the parked pool.  Both measured on the 5-degree IGRF + T96 grid (full grid: 21.8 s with
+// 4 and 1, 17.5 s with 32 and 64 when workers held four; 16, 24, 40 and 48 lanes within
+// 0.3 s of 32 with eight).
 constexpr int COOP_LANES_DEFAULT = 32;
 constexpr long long COOP_IDLE_WEIGHT = 64;
 // After the drain, a trajectory that has run this many per-lane steps is handed over on
