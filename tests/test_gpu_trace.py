@@ -203,3 +203,19 @@ def test_sh_ts05_site_fp64(cor, oracle):
     tc, cc = oracle.trace_scan(osc, L.models(internal=0, external=2, crustal=1, chaos=(g, h)), po, 20000)
     print("sh+ts05", cg[0], "cpu", cc)
     compare(row[0], tg, cg[0], tc, cc, rc_tol=0.25)
+
+
+@pytest.mark.parametrize("precision,tail", [(0, 0), (1, -1)], ids=["fp64", "mixed-per-lane"])
+def test_queue_order_leaves_results_unchanged(cor, precision, tail):
+    """The queue order only decides when a trajectory runs, not what it computes: FP64 and
+    the per-lane mixed path give bit-identical per-trajectory records and cutoffs in
+    rigidity order (0) and longest-expected-first order (1, the benchmark's)."""
+    sites = cor.grid_sites()[::211]
+    rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
+    ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
+    out = [cor.trace(ms, p, rows, step_limit=100000, precision=precision, tail_lanes=tail, queue_order=q)
+           for q in (0, 1)]
+    (t0, c0), (t1, c1) = out
+    assert np.array_equal(t0, t1) and np.array_equal(c0, c1)
+    print(len(rows), "scans,", len(t0), "trajectories identical in both orders")
