@@ -29,7 +29,7 @@ RE_M = 6.3712e06
 # every symbol include/cor.h and include/cor_gpu.h declare
 EXPORTED = [
     "get_model", "calculate_field", "sph_to_car", "car_to_sph", "geo_to_gsm", "gsm_to_geo",
-    "igrf14_recalc", "read_cutoff_infile_header", "write_cutoff_outfile_header",
+    "internal_recalc", "read_cutoff_infile_header", "write_cutoff_outfile_header",
     "read_field_infile_header", "write_field_outfile_header", "read_ts05_params",
     "read_t96_params", "trajectory_simulation_cutoff", "field_simulation",
     "internal_models", "external_models", "crustal_models",
@@ -38,8 +38,20 @@ EXPORTED = [
     "cor_trace_plan_stats", "cor_trace_plan_destroy", "cor_gpu_trace", "cor_gpu_field",
     "cor_scan_count", "cor_geodetic_to_geocentric", "cor_direction_to_geo", "cor_gpu_synchronize",
     "cor_trace_plan_set_epoch", "cor_gpu_trace_series", "cor_refine_scan", "cor_gpu_trace_tracks",
-    "trajectory_simulation",
+    "trajectory_simulation", "calculate_position_and_speed", "string_to_uint64", "string_to_double",
+    "cor_gpu_rk_step", "IGRF_9_COEFS", "IGRF_10_COEFS", "IGRF_11_COEFS", "IGRF_12_COEFS",
+    "IGRF_13_COEFS", "IGRF_14_COEFS", "HIST_COEFS", "CALS10K_2_COEFS",
 ]
+
+# internal model versions (src/models.c:110) -> descriptor symbols
+INTERNAL_TABLES = {"igrf14": "IGRF_14_COEFS", "igrf13": "IGRF_13_COEFS", "igrf12": "IGRF_12_COEFS",
+                   "igrf11": "IGRF_11_COEFS", "igrf10": "IGRF_10_COEFS", "igrf9": "IGRF_9_COEFS",
+                   "hist": "HIST_COEFS", "cals10k.2": "CALS10K_2_COEFS"}
+
+
+class Vector(C.Structure):
+    """src/common.h:13-24 (x, y, z; or r, theta, phi)"""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
 
 
 class ModelsParams(C.Structure):
@@ -114,8 +126,9 @@ def lib():
     L.cor_gpu_error_string.restype = C.c_char_p
     L.cor_gpu_error_string.argtypes = [C.c_int]
     L.cor_gpu_last_hip_error.restype = C.c_char_p
-    L.igrf14_recalc.argtypes = [C.POINTER(RecalcTime)] + [DP] * 6
-    L.igrf14_recalc.restype = C.c_int
+    L.internal_recalc.argtypes = [C.c_void_p, C.POINTER(RecalcTime)] + [DP] * 6
+    L.internal_recalc.restype = C.c_int
+    L.cor_gpu_rk_step.argtypes = [DP, DP, DP, DP, DP, C.c_int64]
     L.cor_scan_count.argtypes = [C.POINTER(Scan)]
     L.cor_scan_count.restype = C.c_int64
     L.cor_trace_plan_create.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p,
@@ -160,14 +173,16 @@ def synchronize():
     check(lib().cor_gpu_synchronize(), "cor_gpu_synchronize")
 
 
-def recalc(iy, mes=1, ide=1, id_=1, ih=0, mi=0, is_=0, parmod=None):
-    """igrf14_recalc on the host -> ModelsParams (PARMOD filled from `parmod`,
-    in the reference order Pdyn, Dst, By, Bz, W1..W6)."""
+def recalc(iy, mes=1, ide=1, id_=1, ih=0, mi=0, is_=0, parmod=None, model="igrf14"):
+    """internal_recalc of an internal model (default IGRF-14) on the host ->
+    ModelsParams (PARMOD filled from `parmod`, in the reference order Pdyn, Dst, By,
+    Bz, W1..W6)."""
     p = ModelsParams()
     t = RecalcTime(iy, mes, ide, id_, ih, mi, is_)
-    rc = lib().igrf14_recalc(C.byref(t), p.G, p.H, p.REC, p.A1, p.A2, p.A3)
+    tab = C.addressof(C.c_char.in_dll(lib(), INTERNAL_TABLES[model]))
+    rc = lib().internal_recalc(C.c_void_p(tab), C.byref(t), p.G, p.H, p.REC, p.A1, p.A2, p.A3)
     if rc != 0:
-        raise CorError(f"igrf14_recalc rejected epoch {iy}")
+        raise CorError(f"internal_recalc({model}) rejected epoch {iy}")
     if parmod is not None:
         for k, v in enumerate(parmod):
             p.PARMOD[k] = float(v)

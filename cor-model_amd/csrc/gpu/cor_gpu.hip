@@ -354,6 +354,8 @@ const char *cor_gpu_error_string(int code) {
 
 const char *cor_gpu_last_hip_error(void) { return g_hip_error; }
 
+void cor_set_error(const char *msg) { snprintf(g_hip_error, sizeof g_hip_error, "%s", msg ? msg : ""); }
+
 int cor_gpu_synchronize(void) {
     int rc = ensure_init();
     if (rc) return rc;
@@ -807,6 +809,39 @@ int cor_gpu_trace_tracks(const cor_model_set *models, const ModelsParams *epoch,
     }
     cor_trace_plan_destroy(p);
     return rc;
+}
+
+int cor_gpu_rk_step(const double *h, const double *d, const double *field, double *vel, double *pos, int64_t n) {
+    if (n < 0 || (n > 0 && (!h || !d || !field || !vel || !pos))) return COR_E_INVALID;
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n == 0) return COR_OK;
+    // one block: h, d (n each), field, vel, pos (3n each)
+    const size_t nn = (size_t)n;
+    double *blk = nullptr;
+    hipError_t e = hipMalloc(&blk, sizeof(double) * 11 * nn);
+    RkStepArgs a;
+    a.h = blk;
+    a.d = blk + nn;
+    a.field = blk + 2 * nn;
+    a.vel = blk + 5 * nn;
+    a.pos = blk + 8 * nn;
+    a.n = n;
+    if (e == hipSuccess) e = hipMemcpyAsync(blk, h, sizeof(double) * nn, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(blk + nn, d, sizeof(double) * nn, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(blk + 2 * nn, field, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.vel, vel, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.pos, pos, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) {
+        void *args[] = {&a};
+        e = hipLaunchKernel(rk_step_kernel_fp64(), dim3((unsigned)((n + 255) / 256)), dim3(256), args, 0, g_stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(vel, a.vel, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pos, a.pos, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    (void)hipFree(blk);
+    if (e != hipSuccess) return fail_hip(e, "cor_gpu_rk_step");
+    return COR_OK;
 }
 
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,

@@ -15,6 +15,13 @@ struct FieldBatchArgs {
     int64_t n;
 };
 
+struct RkStepArgs {
+    const double *h, *d;    // n step lengths [s] and 1e-9 zn q / hm
+    const double *field;    // n x 3 GEO field [nT]
+    double *vel, *pos;      // n x 3, advanced in place
+    int64_t n;
+};
+
 // Returns the kernel for (internal, external, crustal) or nullptr when that model set
 // is not compiled in.
 const void *trace_kernel_fp64(int intk, int ext, int cru);
@@ -26,5 +33,6 @@ const void *field_kernel_mixed(int intk, int ext, int cru);
 // tables of an epoch in place (EpochDev::coop).
 const void *field_kernel_coop(int intk, int ext, int cru, int lanes);
 const void *coop_build_kernel();
+const void *rk_step_kernel_fp64();
 
 }  // namespace cor

@@ -1,16 +1,13 @@
 /*
- * internal.c -- per-epoch setup of the IGRF-14 internal field on the host
- * (lib/internal/internal.c:12-180 for the IGRF-14 descriptor,
- * lib/internal/coefs/igrf_14_coefs.h:88-95).  Runs once per epoch in FP64; its
- * outputs (recursion-scaled G/H, REC, GEO->GSM rows) are uploaded to the GPU.
+ * internal.c -- per-epoch setup of the internal field models on the host
+ * (lib/internal/internal.c:12-180, internal_recalc).  Runs once per epoch in FP64; its
+ * outputs (recursion-scaled G/H, REC, GEO->GSM rows) are uploaded to the GPU, whose
+ * IGRF-form synthesis serves every internal model (IGRF-9..14, the historical model
+ * and CALS10k.2: the same degree-13 triangle, different tables).
  */
 #include <math.h>
 #include <stdint.h>
 #include "cor_host.h"
-
-#define IGRF14_STEP 5u
-#define IGRF14_MIN 1900u
-#define IGRF14_MAX 2030u
 
 /* lib/internal/internal.c:38-62: Russell (1971) sun position and sidereal time. */
 static void sun_position(int iy, int iday, int ih, int min, int is, double *gst, double *slong,
@@ -39,15 +36,16 @@ static void sun_position(int iy, int iday, int ih, int min, int is, double *gst,
 }
 
 /* lib/internal/internal.c:12-36,64-87: coefficients at (year, day of year): linear
- * interpolation between 5-year rows, or main field + secular variation (index <= 45)
- * in the last 5 years of the table.  The descriptor's unsigned arithmetic is kept. */
-static int coefs_at(int year, int iday, double *G, double *H) {
-    const uint32_t nrows = COR_IGRF14_NROWS;
-    const double step = (double)IGRF14_STEP;
-    if (IGRF14_MAX - (uint32_t)year <= IGRF14_STEP) {
-        const double *g0 = cor_igrf14_rows[nrows - 4], *h0 = cor_igrf14_rows[nrows - 3];
-        const double *gs = cor_igrf14_rows[nrows - 2], *hs = cor_igrf14_rows[nrows - 1];
-        const double dt = year + (iday - 1) / 365.25 - ((int)IGRF14_MAX - step);
+ * interpolation between the table's rows, or main field + secular variation (index
+ * <= 45) within one step of the last year where the model extrapolates.  The
+ * descriptor's unsigned arithmetic (int year against uint32_t years) is kept, so out
+ * of range and negative years behave as in the reference. */
+static int coefs_at(const InternalModelCoefs *c, int year, int iday, double *G, double *H) {
+    const double step = (double)c->years_step;
+    if (c->extrapolate && c->max_year - (uint32_t)year <= c->years_step) {
+        const double *g0 = c->coefs[c->coefs_size - 4], *h0 = c->coefs[c->coefs_size - 3];
+        const double *gs = c->coefs[c->coefs_size - 2], *hs = c->coefs[c->coefs_size - 1];
+        const double dt = year + (iday - 1) / 365.25 - ((int)c->max_year - step);
         for (int k = 0; k < INTERNAL_COEFS_SIZE; k++) {
             G[k] = g0[k];
             H[k] = h0[k];
@@ -58,14 +56,14 @@ static int coefs_at(int year, int iday, double *G, double *H) {
         }
         return 0;
     }
-    if ((uint32_t)year >= IGRF14_MAX) return -1;
-    const uint32_t back = (uint32_t)year <= IGRF14_MIN
-                              ? IGRF14_MAX / IGRF14_STEP - (IGRF14_MIN + 1) / IGRF14_STEP
-                              : IGRF14_MAX / IGRF14_STEP - (uint32_t)year / IGRF14_STEP;
-    const int row = (int)(nrows - 2 * back - 1);
-    const int edge = (int)(IGRF14_MIN + ((row - 1) / 2) * IGRF14_STEP);
-    const double *gx = cor_igrf14_rows[row - 1], *hx = cor_igrf14_rows[row];
-    const double *gy = cor_igrf14_rows[row + 1], *hy = cor_igrf14_rows[row + 2];
+    if ((uint32_t)year >= c->max_year) return -1;
+    const int back = (uint32_t)year <= c->min_year
+                         ? (int)(c->max_year / c->years_step - (c->min_year + 1) / c->years_step)
+                         : (int)(c->max_year / c->years_step - (uint32_t)year / c->years_step);
+    const int row = (int)(c->coefs_size - 2 * back - 1);
+    const int edge = (int)(c->min_year + ((row - 1) / 2) * c->years_step);
+    const double *gx = c->coefs[row - 1], *hx = c->coefs[row];
+    const double *gy = c->coefs[row + 1], *hy = c->coefs[row + 2];
     const double f2 = (year + (iday - 1) / 365.25 - edge) / step;
     const double f1 = 1. - f2;
     for (int k = 0; k < INTERNAL_COEFS_SIZE; k++) {
@@ -75,16 +73,16 @@ static int coefs_at(int year, int iday, double *G, double *H) {
     return 0;
 }
 
-/* lib/internal/internal.c:89-180 (internal_recalc) for IGRF-14. */
-int igrf14_recalc(const RecalcTime *time, double *G, double *H, double *REC, double *A1,
-                  double *A2, double *A3) {
-    if ((uint32_t)time->iy > IGRF14_MAX || (uint32_t)time->iy < IGRF14_MIN) return -1;
+/* lib/internal/internal.c:89-180. */
+int internal_recalc(const InternalModelCoefs *coefs, const RecalcTime *time, double *G, double *H, double *REC,
+                    double *A1, double *A2, double *A3) {
+    if ((uint32_t)time->iy > coefs->max_year || (uint32_t)time->iy < coefs->min_year) return -1;
     for (int n = 1; n < 15; n++) {
         double den = 2 * n - 1;
         den = den * (den - 2);
         for (int m = 1; m <= n; m++) REC[n * (n - 1) / 2 + m - 1] = ((n - m) * (n + m - 2)) / den;
     }
-    if (coefs_at(time->iy, time->id, G, H) != 0) return -1;
+    if (coefs_at(coefs, time->iy, time->id, G, H) != 0) return -1;
 
     /* Schmidt semi-normalisation -> the recursion's scaling */
     double s = 1.;

@@ -7,6 +7,18 @@
 #include <stdlib.h>
 #include "cor_host.h"
 
+void cor_write_cutoff_lines(FILE *out, const cor_scan *scan, const cor_traj *traj, int64_t n,
+                            const cor_cutoff *cut) {
+    for (int64_t i = 0; i < n; i++) {
+        const cor_traj *t = &traj[i];
+        if (t->status != COR_ALLOWED) continue;
+        const double rig = scan->rig + (int)i * scan->del;   /* src/trajectory.c:14 */
+        fprintf(out, "%10.6f%15.10f%12.6f%10.3f%10.3f%10.3f%10.3f%12.6f%16.2f\n", rig, t->vc, t->r, t->td,
+                t->fei, t->ast, t->asf, t->time, t->length_m / 1000.);
+    }
+    fprintf(out, "  CUTOFF with rigidities P(S),P(C),P(M) are:\n%12.5f%12.5f%12.5f\n\n", cut->rl, cut->ru, cut->rc);
+}
+
 void trajectory_simulation_cutoff(InfileHeader *header, const Model **models, ModelsParams *models_params,
                                   FILE *outfile, uint64_t step_limit, double tu_angle,
                                   double particle_mass, uint32_t parallel) {
@@ -29,14 +41,7 @@ void trajectory_simulation_cutoff(InfileHeader *header, const Model **models, Mo
         free(traj);
         return;
     }
-    for (int64_t i = 0; i < n; i++) {
-        const cor_traj *t = &traj[i];
-        if (t->status != COR_ALLOWED) continue;
-        const double rig = header->rig + (int)i * header->del;   /* src/trajectory.c:14 */
-        fprintf(outfile, "%10.6f%15.10f%12.6f%10.3f%10.3f%10.3f%10.3f%12.6f%16.2f\n", rig, t->vc, t->r, t->td,
-                t->fei, t->ast, t->asf, t->time, t->length_m / 1000.);
-    }
-    fprintf(outfile, "  CUTOFF with rigidities P(S),P(C),P(M) are:\n%12.5f%12.5f%12.5f\n\n", cut.rl, cut.ru, cut.rc);
+    cor_write_cutoff_lines(outfile, &scan, traj, n, &cut);
     free(traj);
 }
 
@@ -80,4 +85,23 @@ void trajectory_simulation(InfileHeader *header, const Model **models, ModelsPar
     }
     free(pts);
     free(counts);
+}
+
+/* src/trajectory.c:275-356: one RK step with the frozen field, on the GPU (the
+ * tracer's FP64 step, so the same bits as a step inside the tracer). */
+void calculate_position_and_speed(double h, double d, Vector geo_field, Vector *car_vel, Vector *geo_pos) {
+    const double f[3] = {geo_field.x, geo_field.y, geo_field.z};
+    double v[3] = {car_vel->x, car_vel->y, car_vel->z};
+    double x[3] = {geo_pos->x, geo_pos->y, geo_pos->z};
+    const int rc = cor_gpu_rk_step(&h, &d, f, v, x, 1);
+    if (rc != COR_OK) {
+        fprintf(stderr, "Error: GPU step failed: %s (%s)\n", cor_gpu_error_string(rc), cor_gpu_last_hip_error());
+        return;
+    }
+    car_vel->x = v[0];
+    car_vel->y = v[1];
+    car_vel->z = v[2];
+    geo_pos->x = x[0];
+    geo_pos->y = x[1];
+    geo_pos->z = x[2];
 }

@@ -1,9 +1,32 @@
 /*
- * utils.c -- coordinate transforms of the reference API (src/utils.c:28-101).
- * Host-side: used for per-epoch setup, grid construction and output geometry.
+ * utils.c -- number parsing and coordinate transforms of the reference API
+ * (src/utils.c:8-101).  Host-side: command-line values, per-epoch setup, grid
+ * construction and output geometry.
  */
+#include <errno.h>
+#include <limits.h>
 #include <math.h>
+#include <stdlib.h>
 #include "cor_host.h"
+
+/* src/utils.c:8-16: the whole string as a base-10 unsigned number.  The failure test
+ * is the reference's (nothing parsed, trailing characters, or an ERANGE result equal to
+ * LONG_MIN / LONG_MAX with errno as the caller left it), so values are accepted and
+ * refused exactly as the reference's command line does. */
+int string_to_uint64(char *str_num, uint64_t *num) {
+    char *end = NULL;
+    *num = strtoul(str_num, &end, 10);
+    const int range = (*num == (uint64_t)LONG_MIN || *num == (uint64_t)LONG_MAX) && errno == ERANGE;
+    return (end == str_num || *end != '\0' || range) ? -1 : 0;
+}
+
+/* src/utils.c:18-26: the whole string as a double, with the same failure test. */
+int string_to_double(char *str_num, double *num) {
+    char *end = NULL;
+    *num = strtod(str_num, &end);
+    const int range = (*num == (double)LONG_MIN || *num == (double)LONG_MAX) && errno == ERANGE;
+    return (end == str_num || *end != '\0' || range) ? -1 : 0;
+}
 
 /* src/utils.c:28-41 */
 void sph_to_car(Vector *sph, Vector *car) {

@@ -11,6 +11,7 @@
 #ifndef COR_H
 #define COR_H
 
+#include <stdbool.h>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -87,16 +88,9 @@ typedef struct {
     double A3[3];
 } ModelsParams;
 
-/* Kind of evaluator behind a Model, used to route it to the GPU kernels. */
-typedef enum {
-    COR_MODEL_INTERNAL_IGRF = 1,
-    COR_MODEL_EXTERNAL_T96 = 2,
-    COR_MODEL_EXTERNAL_TS05 = 3,
-    COR_MODEL_CRUSTAL_SH = 4
-} CorModelKind;
-
-/* src/models.h:19-25, plus the evaluator kind and, for spherical-harmonic crustal
- * tables, the table itself ([nm_max][nm_max] G and H in nT). */
+/* src/models.h:19-25.  The registries' models are routed to the GPU evaluators by
+ * identity (their calc_fun); a caller-defined model has no GPU evaluator and makes
+ * the GPU entry points return COR_E_UNSUPPORTED. */
 typedef struct {
     const char *name;
     const char *version;
@@ -104,10 +98,6 @@ typedef struct {
     int (*setup_fun)(FILE *input_file, const InfileHeader *infile_header,
                      ModelsParams *model_params, const void *setup_data);
     void (*calc_fun)(Vector gsm_pos, Vector *total_gsm_field, const ModelsParams *params);
-    CorModelKind kind;
-    int32_t sh_nm_max;
-    const double *sh_g;
-    const double *sh_h;
 } Model;
 
 /* src/models.c:110-112: NULL-terminated registries */
@@ -129,7 +119,18 @@ void car_to_sph(Vector *car, Vector *sph);
 void geo_to_gsm(Vector *geo, Vector *gsm, const double *A1, const double *A2, const double *A3);
 void gsm_to_geo(Vector *gsm, Vector *geo, const double *A1, const double *A2, const double *A3);
 
-/* lib/internal/internal.h:12-20,31 (IGRF-14 chain) */
+/* lib/internal/coefs/coefs.h:9-16: a model's chain of Gauss-coefficient rows (G and H
+ * alternating per epoch, the secular-variation pair last where it extrapolates). */
+typedef struct {
+    const double **coefs;
+    uint32_t coefs_size;
+    uint32_t years_step;
+    uint32_t min_year;
+    uint32_t max_year;
+    bool extrapolate;
+} InternalModelCoefs;
+
+/* lib/internal/internal.h:12-20 */
 typedef struct {
     int iy;
     int mes;
@@ -139,8 +140,25 @@ typedef struct {
     int min;
     int is;
 } RecalcTime;
-int igrf14_recalc(const RecalcTime *time, double *G, double *H, double *REC, double *A1,
-                  double *A2, double *A3);
+
+/* lib/internal/internal.h:22-29: the internal models' tables */
+extern const InternalModelCoefs IGRF_9_COEFS;
+extern const InternalModelCoefs IGRF_10_COEFS;
+extern const InternalModelCoefs IGRF_11_COEFS;
+extern const InternalModelCoefs IGRF_12_COEFS;
+extern const InternalModelCoefs IGRF_13_COEFS;
+extern const InternalModelCoefs IGRF_14_COEFS;
+extern const InternalModelCoefs HIST_COEFS;
+extern const InternalModelCoefs CALS10K_2_COEFS;
+
+/* lib/internal/internal.h:31.  Epoch setup on the host: recursion-scaled G/H, REC and
+ * the GEO->GSM rows A1..A3; -1 for an epoch outside the model's years. */
+int internal_recalc(const InternalModelCoefs *coefs, const RecalcTime *time, double *G, double *H,
+                    double *REC, double *A1, double *A2, double *A3);
+
+/* src/utils.h:7-8: strtoul / strtod of a whole string, 0 or -1 */
+int string_to_uint64(char *str_num, uint64_t *num);
+int string_to_double(char *str_num, double *num);
 
 /* src/io.h:8-13 */
 int read_cutoff_infile_header(FILE *input_file, InfileHeader *infile_header);
@@ -151,6 +169,12 @@ int write_field_outfile_header(FILE *output_file, InfileHeader *infile_header,
                                const Model **models);
 int read_ts05_params(FILE *input_file, ModelsParams *params);
 int read_t96_params(FILE *input_file, ModelsParams *params);
+
+/* src/trajectory.h:9.  One step of the reference's 5th-order Runge-Kutta scheme with
+ * the field frozen (src/trajectory.c:275-356), on the GPU with the tracer's own FP64
+ * step: h [s], d = 1e-9 zn q / hm, GEO field [nT]; velocity [m/s] and position [m]
+ * are advanced in place.  GPU errors are reported on stderr and leave both unchanged. */
+void calculate_position_and_speed(double h, double d, Vector geo_field, Vector *car_vel, Vector *geo_pos);
 
 /* src/trajectory.h:10.  The whole rigidity scan is traced on the GPU in FP64 and the
  * output is written exactly as the reference writes it.  `parallel` is accepted and

@@ -198,6 +198,13 @@ int cor_gpu_trace_tracks(const cor_model_set *models, const ModelsParams *epoch,
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
                   double *geo_nT, int64_t n, int32_t precision);
 
+/* calculate_position_and_speed (src/trajectory.c:275-356, declared src/trajectory.h:9)
+ * over n states: one RK step each with the tracer's FP64 step.  h[n] [s], d[n]
+ * (1e-9 zn q / hm), field n x 3 GEO [nT]; vel and pos n x 3 [m/s, m] advanced in
+ * place. */
+int cor_gpu_rk_step(const double *h, const double *d, const double *field, double *vel, double *pos,
+                    int64_t n);
+
 /* round((rk - rig)/del) + 1 as src/trajectory.c:7 computes it, or COR_E_INVALID. */
 int64_t cor_scan_count(const cor_scan *scan);
 

@@ -10,6 +10,7 @@ reference sources (make -C oracle ref first):
     python tests/golden/make_golden.py chaos    (the SH synthesis fixtures only)
     python tests/golden/make_golden.py field    (the field-map run only)
     python tests/golden/make_golden.py track    (the trajectory-mode runs only)
+    python tests/golden/make_golden.py internal (every internal model's epoch setup only)
 """
 import sys
 from pathlib import Path
@@ -111,6 +112,55 @@ def chaos_fixtures():
     print("chaos vectors", {k: v.shape for k, v in out.items()})
 
 
+# every internal model (lib/internal/internal.h:22-29) with epochs at and around its
+# table's ends, inside an interpolation interval, in the extrapolation range, and out of
+# range on both sides (the reference returns -1 there)
+INTERNAL_EPOCHS = {
+    "IGRF_14_COEFS": [1900, 1912, 2020, 2024, 2025, 2027, 2030, 2031, 1899],
+    "IGRF_13_COEFS": [1900, 1987, 2015, 2019, 2020, 2022, 2025, 2026, 1899],
+    "IGRF_12_COEFS": [1900, 1955, 2010, 2014, 2015, 2018, 2020, 2021, 1899],
+    "IGRF_11_COEFS": [1900, 1944, 2005, 2009, 2010, 2013, 2015, 2016, 1899],
+    "IGRF_10_COEFS": [1900, 1933, 2000, 2004, 2005, 2008, 2010, 2011, 1899],
+    "IGRF_9_COEFS": [1900, 1922, 1995, 1999, 2000, 2003, 2005, 2006, 1899],
+    "HIST_COEFS": [0, 1, 49, 50, 777, 1490, 1949, 1950, 1999, 2000, -1],
+    "CALS10K_2_COEFS": [0, 1, 49, 50, 333, 1010, 1949, 1950, 1999, 2000, -1],
+}
+INTERNAL_DAYS = [(1, 1, 1, 0, 0, 0), (7, 2, 183, 12, 30, 15)]
+
+
+def internal_fixtures():
+    """internal_recalc for every internal model of the reference at the epochs above:
+    return code, G, H, REC and the GEO->GSM rows; and two cutoff runs of the reference
+    CLI with other internal models (IGRF-13 at 2017, CALS10k.2 at 1000 CE)."""
+    for name, model, date, site in [("igrf13_2017_nk100", "igrf13", (2017, 5, 20, 140, 6, 0, 0),
+                                     (1.00314, 40.0, 250.0)),
+                                    ("cals10k2_1000_nk100", "cals10k.2", (1000, 1, 1, 1, 0, 0, 0),
+                                     (1.00314, 30.0, 15.0))]:
+        inp = HERE / f"{name}.in"
+        r0, lat, lon = site
+        L.write_cutoff_input(inp, 0.5, -1.0, 20.0, r0, lat, lon, lat, lon, date=date, nk1=100, dl=0.05)
+        txt = L.run_ref_cutoff(inp, internal=model, steps=200000)
+        (HERE / f"{name}.out").write_text(txt)
+        print(name, txt.strip().splitlines()[-1])
+    ref = L.RefLib()
+    out = {}
+    for tab, years in INTERNAL_EPOCHS.items():
+        rows = []
+        for y in years:
+            for d in INTERNAL_DAYS:
+                e = (y,) + d
+                rc, p = ref.recalc(*e, table=tab)
+                rows.append(e)
+                k = len(rows) - 1
+                out[f"{tab}_{k}_rc"] = np.array(rc)
+                if rc == 0:
+                    for f in ("G", "H", "REC", "A1", "A2", "A3"):
+                        out[f"{tab}_{k}_{f}"] = np.array(getattr(p, f))
+        out[f"{tab}_epochs"] = np.array(rows)
+    np.savez_compressed(HERE / "internal_vectors.npz", **out)
+    print("internal vectors", len(out), "arrays")
+
+
 def field_fixtures():
     """Field-map runs of the reference CLI (-m field, src/field.c:6-54), input and output
     text: IGRF-14 + T96 on a 20 x 30 degree grid at 1.5 Re, and IGRF-14 + the
@@ -169,6 +219,9 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "chaos":
         chaos_fixtures()
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "internal":
+        internal_fixtures()
+        return
     ref = L.RefLib()
     rng = np.random.default_rng(20240501)
     out = {}
@@ -212,6 +265,7 @@ def main():
     chaos_fixtures()
     field_fixtures()
     track_fixtures()
+    internal_fixtures()
 
 
 if __name__ == "__main__":

@@ -193,10 +193,12 @@ class RefLib:
         lib.chaos_geo.argtypes = [C.c_double] * 3 + [C.c_void_p] + [DP] * 3
         self.igrf14 = C.c_void_p.in_dll(lib, "IGRF_14_COEFS")
 
-    def recalc(self, iy, mes, ide, id_, ih, mi, is_):
+    def recalc(self, iy, mes, ide, id_, ih, mi, is_, table="IGRF_14_COEFS"):
+        """internal_recalc (lib/internal/internal.c:89-180) with one of the reference's
+        coefficient descriptors (lib/internal/internal.h:22-29)."""
         p = OParams()
         t = RecalcTime(iy, mes, ide, id_, ih, mi, is_)
-        addr = C.addressof(C.c_char.in_dll(self.lib, "IGRF_14_COEFS"))
+        addr = C.addressof(C.c_char.in_dll(self.lib, table))
         rc = self.lib.internal_recalc(C.c_void_p(addr), C.byref(t), p.G, p.H, p.REC,
                                       p.A1, p.A2, p.A3)
         return rc, p
@@ -260,6 +262,15 @@ class RefLib:
 
 def have_ref():
     return REF_SO.exists() and REF_BIN.exists()
+
+
+def read_cutoff_header(path):
+    """The numbers of a cutoff input header (src/io.c:7: rig zn rk / r0 the0 fi0 /
+    the1 fi1 / date / nk1 iopt ist del)."""
+    v = Path(path).read_text().split()
+    return {"rig": float(v[0]), "zn": float(v[1]), "rk": float(v[2]), "r0": float(v[3]), "the0": float(v[4]),
+            "fi0": float(v[5]), "the1": float(v[6]), "fi1": float(v[7]), "date": tuple(int(x) for x in v[8:15]),
+            "nk1": int(v[15]), "del": float(v[18])}
 
 
 def write_cutoff_input(path, rig, zn, rk, r0, the0, fi0, the1, fi1, date=(2020, 1, 1, 1, 0, 0, 0),

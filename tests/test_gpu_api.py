@@ -13,6 +13,7 @@ rigidities are identical; from 1 GV above the band every printed column agrees (
 map: header identical, every value within 0.01 nT of the printed reference value.
 """
 import ctypes as C
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -23,6 +24,7 @@ import oracle_lib as L
 
 pytestmark = pytest.mark.gpu
 
+REPO = Path(__file__).resolve().parents[1]
 GOLD = Path(__file__).resolve().parent / "golden"
 libc = C.CDLL(None)
 libc.fopen.restype = C.c_void_p
@@ -42,8 +44,7 @@ class InfileHeader(C.Structure):
 
 class Model(C.Structure):
     _fields_ = [("name", C.c_char_p), ("version", C.c_char_p), ("setup_data", C.c_void_p),
-                ("setup_fun", SETUP), ("calc_fun", C.c_void_p), ("kind", C.c_int), ("sh_nm_max", C.c_int32),
-                ("sh_g", C.c_void_p), ("sh_h", C.c_void_p)]
+                ("setup_fun", SETUP), ("calc_fun", C.c_void_p)]
 
 
 def lib():
@@ -126,19 +127,17 @@ CUTOFF_RUNS = [
 ]
 
 
-@pytest.mark.parametrize("name,internal,external,steps,rc_tol", CUTOFF_RUNS, ids=[r[0] for r in CUTOFF_RUNS])
-def test_trajectory_simulation_cutoff_text(tmp_path, name, internal, external, steps, rc_tol):
-    A.init(0)
-    L_ = lib()
-    mine, h = run_cutoff(L_, tmp_path, name, internal, external, steps)
-    ref = (GOLD / f"{name}.out").read_text()
+def compare_cutoff_text(name, mine, ref, rig, dl, rc_tol):
+    """The reference's cutoff text against this build's: identical headers, cutoffs
+    within rc_tol, the same allowed rigidities outside the union penumbra band, and the
+    per-rigidity columns of the allowed lines above the band."""
     assert header_part(mine) == header_part(ref)
     got, cut_g = L.parse_cutoff_output(mine)
     want, cut_r = L.parse_cutoff_output(ref)
     print(name, "gpu", cut_g, "reference", cut_r)
     assert all(abs(a - b) <= rc_tol + 1e-9 for a, b in zip(cut_g, cut_r)), (cut_g, cut_r)
     lo, hi = min(cut_g[0], cut_r[0]), max(cut_g[1], cut_r[1])
-    key = lambda r: int(round((r - h.rig) / h.del_))        # noqa: E731
+    key = lambda r: int(round((r - rig) / dl))        # noqa: E731
     g_out = {key(r["rig"]) for r in got if r["rig"] < lo - 1e-9 or r["rig"] > hi + 1e-9}
     r_out = {key(r["rig"]) for r in want if r["rig"] < lo - 1e-9 or r["rig"] > hi + 1e-9}
     assert g_out == r_out, sorted(g_out ^ r_out)[:10]
@@ -157,6 +156,31 @@ def test_trajectory_simulation_cutoff_text(tmp_path, name, internal, external, s
             assert abs(g[c] - r[c]) <= 1e-5 * abs(r[c]) + 0.011, (r["rig"], c, g[c], r[c])
         n += 1
     print(name, n, "lines compared column by column")
+
+
+@pytest.mark.parametrize("name,internal,external,steps,rc_tol", CUTOFF_RUNS, ids=[r[0] for r in CUTOFF_RUNS])
+def test_trajectory_simulation_cutoff_text(tmp_path, name, internal, external, steps, rc_tol):
+    A.init(0)
+    L_ = lib()
+    mine, h = run_cutoff(L_, tmp_path, name, internal, external, steps)
+    compare_cutoff_text(name, mine, (GOLD / f"{name}.out").read_text(), h.rig, h.del_, rc_tol)
+
+
+RELINKED = REPO / "oracle" / "_ref" / "cor_relinked"
+
+
+@pytest.mark.skipif(not RELINKED.exists(), reason="oracle/_ref/cor_relinked is not built (make -C oracle relink)")
+@pytest.mark.parametrize("name,internal,external,steps,rc_tol", CUTOFF_RUNS[:2], ids=[r[0] for r in CUTOFF_RUNS[:2]])
+def test_reference_cli_relinked_on_this_library(name, internal, external, steps, rc_tol):
+    """The reference's own command line (src/main.c, src/args.c, compiled from its
+    sources and linked against libcor_amd.so instead of its own library) on the
+    golden inputs, as the golden outputs were made: -p -i igrf14 [-e t96] -s STEPS."""
+    cmd = [str(RELINKED), "-p", "-i", internal] + (["-e", external] if external else []) + ["-s", str(steps),
+                                                                                          str(GOLD / f"{name}.in")]
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    h = L.read_cutoff_header(GOLD / f"{name}.in")
+    compare_cutoff_text(name, res.stdout, (GOLD / f"{name}.out").read_text(), h["rig"], h["del"], rc_tol)
 
 
 FIELD_RUNS = [("field_igrf_t96", "t96", None, 9 * 12), ("field_igrf_chaos", None, "chaos7.16", 7 * 6)]
@@ -236,8 +260,12 @@ def test_trajectory_simulation_tracks(tmp_path, name, external, steps):
     finally:
         libc.fclose(g1)
         libc.fclose(g2)
-    for kind in ("tragsm", "trasph"):
-        mine = np.loadtxt(outs[kind], ndmin=2)
+    compare_tracks(name, outs["tragsm"], outs["trasph"])
+
+
+def compare_tracks(name, gsm_file, sph_file):
+    for kind, path in (("tragsm", gsm_file), ("trasph", sph_file)):
+        mine = np.loadtxt(path, ndmin=2)
         ref = np.loadtxt(GOLD / f"{name}.{kind}", ndmin=2)
         assert mine.shape == ref.shape, (kind, mine.shape, ref.shape)
         assert np.array_equal(mine[:, 0], ref[:, 0])
@@ -250,3 +278,99 @@ def test_trajectory_simulation_tracks(tmp_path, name, external, steps):
         print(name, kind, len(ref), "lines; max deviation beyond the printed unit / scale: first half",
               excess[:half].max(), "all", excess.max())
         assert excess[:half].max() <= 1e-9 and excess.max() <= 1e-6
+
+
+# ---------------------------------------------------------------- the `cor` command line
+
+COR = REPO / "cor-model_amd" / "cor"
+needs_cli = pytest.mark.skipif(not COR.exists(), reason="cor-model_amd/cor is not built")
+
+
+def cor(*args, cwd=None):
+    res = subprocess.run([str(COR), *map(str, args)], capture_output=True, text=True, timeout=300, cwd=cwd)
+    assert res.returncode == 0, res.stderr
+    return res.stdout
+
+
+@needs_cli
+def test_cli_cutoff_matches_reference_text():
+    """cor -p -i igrf14 -s 1000000 INFILE: the reference's command line and output for
+    config 1 (FP64 IGRF is bit-identical, so the cutoffs are equal)."""
+    name = "config1_igrf_nk100"
+    h = L.read_cutoff_header(GOLD / f"{name}.in")
+    out = cor("-p", "-i", "igrf14", "-s", 1000000, GOLD / f"{name}.in")
+    compare_cutoff_text(name, out, (GOLD / f"{name}.out").read_text(), h["rig"], h["del"], 0.0)
+
+
+INTERNAL_RUNS = [("igrf13_2017_nk100", "igrf13"), ("cals10k2_1000_nk100", "cals10k.2")]
+
+
+@needs_cli
+@pytest.mark.parametrize("name,model", INTERNAL_RUNS, ids=[r[0] for r in INTERNAL_RUNS])
+def test_cli_other_internal_models(name, model):
+    """-i igrf13 at 2017 and -i cals10k.2 at 1000 CE against the reference's runs of the
+    same inputs: the other internal models' epoch setup (bit-exact on the host) drives
+    the same GPU synthesis, so the FP64 cutoffs are equal."""
+    h = L.read_cutoff_header(GOLD / f"{name}.in")
+    out = cor("-p", "-i", model, "-s", 200000, GOLD / f"{name}.in")
+    compare_cutoff_text(name, out, (GOLD / f"{name}.out").read_text(), h["rig"], h["del"], 0.0)
+
+
+@needs_cli
+def test_cli_batch_of_input_files(tmp_path):
+    """Several INFILEs in one invocation (one GPU launch for the files that share the
+    epoch and T96 inputs): each output equals the single-file run's bar against the
+    reference's text."""
+    names = ["t96_equator_nk400", "t96_mid_nk400"]
+    cor("-p", "-i", "igrf14", "-e", "t96", "-s", 1000000, "-o", tmp_path, *[GOLD / f"{n}.in" for n in names])
+    for name in names:
+        h = L.read_cutoff_header(GOLD / f"{name}.in")
+        mine = (tmp_path / f"{name}.in.out").read_text()
+        compare_cutoff_text(name, mine, (GOLD / f"{name}.out").read_text(), h["rig"], h["del"], 0.05)
+
+
+@needs_cli
+def test_cli_field_and_trajectory_modes(tmp_path):
+    """-m field and -m trajectory through the command line, against the reference's
+    field map and track files."""
+    out = cor("-m", "field", "-i", "igrf14", "-e", "t96", GOLD / "field_igrf_t96.in").splitlines()
+    ref = (GOLD / "field_igrf_t96.out").read_text().splitlines()
+    assert len(out) == len(ref)
+    for a, b in zip(out, ref):
+        try:
+            va, vb = np.array(a.split(), dtype=float), np.array(b.split(), dtype=float)
+        except ValueError:
+            assert a == b
+            continue
+        assert va.shape == vb.shape and np.all(np.abs(va - vb) <= 0.0100001), (a, b)
+    cor("-m", "trajectory", "-i", "igrf14", "-e", "t96", "-s", 20000, "-o", "t.txt", GOLD / "track_t96_5gv.in",
+        cwd=tmp_path)
+    compare_tracks("track_t96_5gv", tmp_path / "tragsm_t.txt", tmp_path / "trasph_t.txt")
+
+
+def test_calculate_position_and_speed_is_the_reference_step():
+    """calculate_position_and_speed (src/trajectory.c:275-356) through the C API, on the
+    GPU, bit for bit against the oracle's restatement of the step (itself pinned by the
+    reference's golden cutoff runs and the 5th-order convergence closed form), over
+    seeded states of the size the tracer meets (Earth radii, MeV..GeV protons, fields
+    of tens of nT to tens of microtesla)."""
+    L_ = A.lib()
+    L_.calculate_position_and_speed.restype = None
+    L_.calculate_position_and_speed.argtypes = [C.c_double, C.c_double, A.Vector, C.POINTER(A.Vector),
+                                                C.POINTER(A.Vector)]
+    o = L.Oracle()
+    o.lib.o_rk_step.argtypes = [C.c_double, C.c_double, L.DP, L.DP, L.DP]
+    o.lib.o_rk_step.restype = None
+    rng = np.random.default_rng(275)
+    A.init(0)
+    for _ in range(200):
+        h = 10.0 ** rng.uniform(-6, -2)
+        d = -(10.0 ** rng.uniform(-1.5, 0.5))
+        f = rng.normal(size=3) * 10.0 ** rng.uniform(1, 4.5)
+        v = rng.normal(size=3) * 1e8
+        x = rng.normal(size=3) * 2e7
+        vg, xg = A.Vector(*v), A.Vector(*x)
+        L_.calculate_position_and_speed(h, d, A.Vector(*f), C.byref(vg), C.byref(xg))
+        vo, xo = v.copy(), x.copy()
+        o.lib.o_rk_step(h, d, f.ctypes.data_as(L.DP), vo.ctypes.data_as(L.DP), xo.ctypes.data_as(L.DP))
+        assert (vg.x, vg.y, vg.z) == tuple(vo) and (xg.x, xg.y, xg.z) == tuple(xo)

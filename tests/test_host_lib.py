@@ -31,6 +31,8 @@ def declared_functions():
                 names.add(m.group(1))
         for m in re.finditer(r"^extern const Model \*(\w+)\[\]", text, re.M):
             names.add(m.group(1))
+        for m in re.finditer(r"^extern const InternalModelCoefs (\w+);", text, re.M):
+            names.add(m.group(1))
     return names
 
 
@@ -41,6 +43,86 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
     assert set(A.EXPORTED) <= names
+
+
+def test_internal_recalc_every_model_matches_reference_fixtures():
+    """internal_recalc (lib/internal/internal.c:89-180) for IGRF-9..14, the historical
+    model and CALS10k.2, bit for bit against the reference at epochs around each table's
+    ends, inside interpolation intervals and in the extrapolation range; out-of-range
+    years fail as the reference's do."""
+    vec = np.load(GOLD / "internal_vectors.npz")
+    lib = A.lib()
+    n_ok = 0
+    for tab in A.INTERNAL_TABLES.values():
+        addr = C.addressof(C.c_char.in_dll(lib, tab))
+        for k, e in enumerate(vec[f"{tab}_epochs"]):
+            p = A.ModelsParams()
+            t = A.RecalcTime(*[int(v) for v in e])
+            rc = lib.internal_recalc(C.c_void_p(addr), C.byref(t), p.G, p.H, p.REC, p.A1, p.A2, p.A3)
+            assert rc == int(vec[f"{tab}_{k}_rc"]), (tab, tuple(e))
+            if rc == 0:
+                n_ok += 1
+                for f in ("G", "H", "REC", "A1", "A2", "A3"):
+                    assert np.array_equal(np.array(getattr(p, f)), vec[f"{tab}_{k}_{f}"]), (tab, tuple(e), f)
+    assert n_ok >= 100
+
+
+def test_internal_models_registry_matches_reference():
+    """src/models.c:22-112: the eight internal models by version, with their names."""
+    lib = A.lib()
+    get = lib.get_model
+    get.restype = C.c_void_p
+    get.argtypes = [C.c_void_p, C.c_char_p]
+    reg = C.addressof(C.c_char.in_dll(lib, "internal_models"))
+    names = {"igrf14": "IGRF-14", "igrf13": "IGRF-13", "igrf12": "IGRF-12", "igrf11": "IGRF-11",
+             "igrf10": "IGRF-10", "igrf9": "IGRF-9", "hist": "Historical (0-1968 CE)",
+             "cals10k.2": "Historical (CALS10k.2)"}
+    for ver, name in names.items():
+        m = get(reg, ver.encode())
+        assert m, ver
+        assert C.cast(C.c_void_p.from_address(m).value, C.c_char_p).value.decode() == name
+    assert not get(reg, b"igrf15")
+
+
+def test_string_parsers_follow_reference():
+    """src/utils.c:8-26: whole-string parses; the command line's -s and -u values."""
+    lib = A.lib()
+    lib.string_to_uint64.argtypes = [C.c_char_p, C.POINTER(C.c_uint64)]
+    lib.string_to_double.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
+    u = C.c_uint64()
+    d = C.c_double()
+    for txt, rc, val in [(b"1000000", 0, 1000000), (b"25000", 0, 25000), (b"12x", -1, None), (b"", -1, None),
+                         (b" 7", 0, 7), (b"-1", 0, 2 ** 64 - 1), (b"18446744073709551615", 0, 2 ** 64 - 1)]:
+        assert lib.string_to_uint64(txt, C.byref(u)) == rc, txt
+        if val is not None:
+            assert u.value == val, txt
+    for txt, rc, val in [(b"0.01", 0, 0.01), (b"1e-3", 0, 1e-3), (b"0.01rad", -1, None), (b"", -1, None)]:
+        assert lib.string_to_double(txt, C.byref(d)) == rc, txt
+        if val is not None:
+            assert d.value == val, txt
+
+
+def test_caller_defined_model_is_refused():
+    """A Model the registries do not hold has no GPU evaluator: the model list is
+    refused before any device work (the reference's five-field struct is all that is
+    read)."""
+    lib = A.lib()
+
+    class Model(C.Structure):
+        _fields_ = [("name", C.c_char_p), ("version", C.c_char_p), ("setup_data", C.c_void_p),
+                    ("setup_fun", C.c_void_p), ("calc_fun", C.c_void_p)]
+
+    CALC = C.CFUNCTYPE(None, A.Vector, C.POINTER(A.Vector), C.c_void_p)
+    fn = CALC(lambda pos, tot, par: None)
+    m = Model(b"my model", b"mine", None, None, C.cast(fn, C.c_void_p))
+    models = (C.c_void_p * 2)(C.addressof(m), None)
+    lib.calculate_field.argtypes = [A.Vector, C.POINTER(A.Vector), C.c_void_p, C.POINTER(A.ModelsParams)]
+    lib.calculate_field.restype = None
+    out = A.Vector()
+    p = A.recalc(2020)
+    lib.calculate_field(A.Vector(1e7, 0, 0), C.byref(out), models, C.byref(p))
+    assert np.isnan(out.x) and np.isnan(out.y) and np.isnan(out.z)
+    assert b"my model" in lib.cor_gpu_last_hip_error()
 
 
 def test_igrf14_recalc_matches_reference_fixtures():
@@ -202,22 +284,27 @@ def test_refine_scan():
 
 
 def chaos716_tables():
-    """The crustal registry's CHAOS-7.16 tables as the library holds them ([186][186])."""
+    """The crustal registry's CHAOS-7.16 tables as the library holds them: the stored
+    triangle (m <= n, row after row) expanded to [186][186] as the GPU receives it."""
     lib = A.lib()
 
     class Model(C.Structure):
         _fields_ = [("name", C.c_char_p), ("version", C.c_char_p), ("setup_data", C.c_void_p),
-                    ("setup_fun", C.c_void_p), ("calc_fun", C.c_void_p), ("kind", C.c_int),
-                    ("sh_nm_max", C.c_int32), ("sh_g", C.POINTER(C.c_double)), ("sh_h", C.POINTER(C.c_double))]
+                    ("setup_fun", C.c_void_p), ("calc_fun", C.c_void_p)]
 
     lib.get_model.restype = C.c_void_p
     lib.get_model.argtypes = [C.c_void_p, C.c_char_p]
     m = lib.get_model(C.addressof(C.c_void_p.in_dll(lib, "crustal_models")), b"chaos7.16")
     assert m
     md = Model.from_address(m)
-    nm = md.sh_nm_max
-    g = np.ctypeslib.as_array(md.sh_g, shape=(nm, nm)).copy()
-    h = np.ctypeslib.as_array(md.sh_h, shape=(nm, nm)).copy()
+    nm = 186
+    ntri = nm * (nm + 1) // 2
+    tri = np.ctypeslib.as_array((C.c_double * (2 * ntri)).in_dll(lib, "cor_chaos716_tri")).reshape(2, ntri)
+    g = np.zeros((nm, nm))
+    h = np.zeros((nm, nm))
+    rows, cols = np.tril_indices(nm)
+    g[rows, cols] = tri[0]
+    h[rows, cols] = tri[1]
     return md.name.decode(), g, h
 
 
@@ -249,3 +336,26 @@ def test_t96_derived_tables_match_generator():
     tabs = dict(gen.derived())
     assert abs(tabs["k_shl_sq"][0] - (1 / 12.75434981 ** 2 + 1 / 1.752483754 ** 2) ** 0.5) < 1e-15
     assert abs(tabs["k_shl_rp"][0] * 12.75434981 - 1) < 1e-15 and abs(tabs["k_shl_rk"][0] * 1.752483754 - 1) < 1e-15
+
+
+REF_SRC = Path("/root/reference/src")
+
+
+@pytest.mark.skipif(not (REF_SRC / "main.c").exists(), reason="reference sources not present")
+def test_reference_cli_links_against_library(tmp_path):
+    """The reference's command line (src/main.c + src/args.c, with its own headers,
+    unedited) compiles and links against libcor_amd.so alone: every symbol its callers
+    use (string_to_uint64, string_to_double, the registries, get_model, the io
+    functions, the three simulation entry points) is exported with the reference's
+    signatures.  Only --help and --version are run here (no GPU)."""
+    import subprocess
+    lib_dir = REPO / "cor-model_amd"
+    exe = tmp_path / "cor_relinked"
+    cmd = ["gcc", "-std=c17", "-O1", "-w", f"-I{REF_SRC}", f"-I{REF_SRC.parent / 'lib'}", "-o", str(exe),
+           str(REF_SRC / "main.c"), str(REF_SRC / "args.c"), f"-L{lib_dir}", "-lcor_amd", f"-Wl,-rpath,{lib_dir}"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    out = subprocess.run([str(exe), "--version"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "cor-model 3.0" in out.stdout
+    out = subprocess.run([str(exe), "--help"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "--internal-field" in out.stdout
