@@ -154,11 +154,11 @@ def main():
     ap.add_argument("--nk1", type=int, default=400)
     ap.add_argument("--step-limit", type=int, default=1000000)
     ap.add_argument("--queue-order", type=int, default=1)
-    ap.add_argument("--tail-lanes", type=int, default=0,
-                    help="cooperative tail threshold (0 = library default, -1 = per-lane mode only)")
+    ap.add_argument("--coop", type=int, default=0,
+                    help="cooperative mode (0 = on, the library default; -1 = per-lane mode only)")
     ap.add_argument("--handover-steps", type=int, default=0,
-                    help="per-lane steps after which a trajectory is handed to the cooperative mode "
-                         "on its own once the queue is drained (0: library default, negative: off)")
+                    help="accepted per-lane steps before a trajectory continues in the cooperative "
+                         "mode (0: library default)")
     ap.add_argument("--fp64-stride", type=int, default=32,
                     help="FP64 comparison run on every fp64-stride-th site of the benchmarked sites")
     ap.add_argument("--no-fp64", action="store_true")
@@ -183,7 +183,7 @@ def main():
     params = A.recalc(*EPOCH, parmod=PARMOD)
     models = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
     mixed = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_MIXED,
-                   queue_order=args.queue_order, tail_lanes=args.tail_lanes,
+                   queue_order=args.queue_order, coop=args.coop,
                    handover_steps=args.handover_steps)
     n = mixed.n
     print(f"[bench] {len(scans)} sites, {n} trajectories; warmup {args.warmup}, steps {args.steps}",
@@ -219,7 +219,7 @@ def main():
         # the mixed path on the same sites, for a like-for-like speed ratio (both runs
         # of a site subset end with the same slowest trajectories)
         msub = A.Plan(models, params, scans[sub], step_limit=args.step_limit, precision=A.PREC_MIXED,
-                      queue_order=args.queue_order, tail_lanes=args.tail_lanes,
+                      queue_order=args.queue_order, coop=args.coop,
                       handover_steps=args.handover_steps)
         t_msub, _ = time_plan(msub, 1, 0)
         msub.close()
@@ -277,7 +277,7 @@ def main():
                                "(Pdyn 2, Dst -20, By 0, Bz -2), 0.01-20 GV step 0.01",
                    "sites": len(scans), "site_stride": args.stride, "trajectories": int(n),
                    "nk1": args.nk1, "step_limit": args.step_limit, "tu_angle": 0.01,
-                   "queue_order": args.queue_order, "tail_lanes": args.tail_lanes,
+                   "queue_order": args.queue_order, "coop": args.coop,
                    "handover_steps": args.handover_steps},
         "kernel_ms": k_mixed,
         "integration_steps_per_s": st_m.steps / (k_mixed * 1e-3),

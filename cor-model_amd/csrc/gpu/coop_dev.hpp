@@ -23,6 +23,7 @@
 // whose terms cancel by four orders of magnitude) are summed in a form without the
 // cancellation (see the lane-table builder).
 #pragma once
+#include <utility>
 #include "dev_types.hpp"
 #include "igrf_dev.hpp"
 #include "t96_dev.hpp"
@@ -329,43 +330,66 @@ __device__ __forceinline__ void group_sum(float *p) {
     }
 }
 
-// This lane's rows of the lane tables, read together at the top of a step so their LDS
-// latencies overlap (rows past a family's size are valid zero or unused rows).
-struct CoopLane {
-    float4 harm, cone, cyl, t87, conic;
-    float icon[24], circ[16], sheet[24];
+// ---------------------------------------------------------------- canonical order
+//
+// The field of a trajectory is a sum of terms of several families.  So that a
+// trajectory's bits do not depend on how many lanes evaluate it, every sum has one
+// canonical order for all group widths: term t of a family lives in slot brev6(t) of
+// 64 slots (the 6-bit reversal of t), a slot's value is its terms added in family order,
+// and the field is the balanced pairwise sum of the 64 slots in slot order.  A group of G
+// lanes gives lane gl the contiguous slots gl*64/G .. (gl+1)*64/G - 1, i.e. the terms
+// t = brev(gl) + G*p (p = 0 .. 64/G - 1, slot brev(p) of the lane's block): the lane
+// sums its block pairwise and the cross-lane steps (balanced in lane order) finish the
+// same tree.  Floating-point addition being commutative, each pairwise step gives the
+// same bits in either operand order, so every lane of a group ends with the total.
+
+constexpr int ilog2c(int g) { return g <= 1 ? 0 : 1 + ilog2c(g / 2); }
+constexpr int brevc(int v, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; i++) r |= ((v >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+// brev(gl) over log2(G) bits (the lane's first term)
+template <int G>
+__device__ __forceinline__ int lane_residue(int gl) {
+    constexpr int L = ilog2c(G);
+    if constexpr (L == 0) return 0;
+    else return (int)(__builtin_bitreverse32((unsigned)gl) >> (32 - L));
+}
+
+// Pairwise sum of a lane's slots pushed in slot order J (a template argument, so every
+// index below is a compile-time constant and the levels stay in registers): level k
+// holds the pending left half of size 2^k.
+template <class V, int NP>
+struct SlotTree {
+    V lv[ilog2c(NP) + 1];
+    template <int J, int K = 0>
+    __device__ __forceinline__ void push(V v) {
+        if constexpr ((J >> K) & 1) {
+            push<J, K + 1>(lv[K] + v);
+        } else {
+            lv[K] = v;
+        }
+    }
+    __device__ __forceinline__ V total() const { return lv[ilog2c(NP)]; }
+};
+
+// f(integral_constant<int, J>) for J = 0 .. N-1, in order
+template <int... J, class Fn>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, J...>, Fn &&f) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+    static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+struct F3 {
+    float x, y, z;
+    __device__ __forceinline__ F3 operator+(const F3 &o) const { return {x + o.x, y + o.y, z + o.z}; }
 };
 
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
-__device__ __forceinline__ void put4(float *d, float4 v) {
-    d[0] = v.x;
-    d[1] = v.y;
-    d[2] = v.z;
-    d[3] = v.w;
-}
-
-// gl: the lane's index in its group.  Families with more terms than the group has
-// lanes (groups of 16 or 32: the Cartesian harmonics and the current loops; groups of 8
-// also the conical harmonics and the Region-2 sheet) are read in their passes instead.
-template <int G>
-__device__ __forceinline__ void load_lane(const CoopTab &T, int gl, CoopLane &L) {
-    if constexpr (G == 64) L.harm = ld4(T.harm[gl]);
-    L.cone = ld4(T.cone[gl & 7]);
-    L.cyl = ld4(T.cyl[gl & 7]);
-    L.t87 = ld4(T.t87[gl & 3]);
-    if constexpr (G >= 16) L.conic = ld4(T.conic[gl & 15]);
-#pragma unroll
-    for (int j = 0; j < 6; j++) put4(L.icon + 4 * j, ld4(&T.icon[gl & 3][4 * j]));
-    if constexpr (G == 64) {
-        const int ci = gl < 24 ? gl : 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) put4(L.circ + 4 * j, ld4(&T.circ[ci][4 * j]));
-    }
-    if constexpr (G >= 16) {
-#pragma unroll
-        for (int j = 0; j < 6; j++) put4(L.sheet + 4 * j, ld4(&T.sheet[gl & 15][4 * j]));
-    }
-}
 
 // One term of the harmonic form shared by the shield sets, birk1shld, birk2shl and the
 // interconnection field: c e^{x sq} (-sq cos(y rp) sin(z rk), rp sin(y rp) sin(z rk),
@@ -389,6 +413,256 @@ struct T96Share {
     float sigma, oimfy, oimfz;
 };
 
+// The point and region state a T96 slot's terms need (uniform per trajectory).
+struct T96Geo {
+    float xx, yy, zz;               // the point scaled by xappa
+    float X_i, Y_i, Z_i;            // the IMF-rotated point (interconnection field)
+    float lx, ly, lz, cxp, cyp, czp;   // Region-1 loop-basis and conical-basis points
+    float wl, wc, wo, ws, wi;       // blend weights: Region-1 loop, cone; Region-2 outer, sheet, inner
+    float xks;                      // Region-2 xi
+    bool need_loop, need_cone;
+};
+
+// One Region-1 dipole (lib/t96/t96.c:809-852, 920-1028): 30574 w (3 (M.r) r - r^2 M) / r^5.
+__device__ __forceinline__ F3 coop_dipole(const CoopTab &T, int s, float w, float qx0, float qy0, float qz0) {
+    typedef float F;
+    const float4 dp = ld4(&T.dip[s][0]);
+    const float4 mm = ld4(&T.dip[s][4]);
+    const F X = qx0 - dp.x, Y = qy0 - dp.y, Z = qz0 - dp.z;
+    const F r2 = X * X + Y * Y + Z * Z;
+    const F qd = w * F(30574.0) / (r2 * r2 * Dm<F>::sqrt(r2));
+    const F md = F(3.0) * (mm.x * X + mm.y * Y + mm.z * Z);
+    return F3{qd * (md * X - r2 * mm.x), qd * (md * Y - r2 * mm.y), qd * (md * Z - r2 * mm.z)};
+}
+
+// The terms of slot t (all families, in family order).  `base` is the smallest t of the
+// pass (a compile-time constant), so families whose terms all lie below it drop out.
+__device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const CoopTab &T, const T96Geo gm, const int base,
+                                            const int t) {
+    using namespace t96;
+    typedef float F;
+    F px = 0, py = 0, pz = 0;
+    // family 1: Cartesian harmonics (47)
+    if (base < CO_NHARM && t < CO_NHARM) {
+        const float4 h = ld4(T.harm[t]);
+        const v3<F> b = co_harm<F>(gm.xx, gm.yy, gm.zz, h.x, h.y, h.z, h.w);
+        px += b.x;
+        py += b.y;
+        pz += b.z;
+    }
+    // family 2: interconnection harmonics (3, one y harmonic each) at the
+    // IMF-rotated point, rotated back (lib/t96/t96.c:66-72)
+    if (base < CO_NICON && t < CO_NICON) {
+        const float *c = T.icon[t];
+        F sy, cy, sz, cz, s2, c2;
+        Dm<F>::sincos(gm.Y_i * c[0], &sy, &cy);
+        Dm<F>::sincos(gm.Z_i * c[1], &sz, &cz);
+        Dm<F>::sincos(gm.Z_i * c[6], &s2, &c2);
+        const F e0 = c[3] * Dm<F>::exp(gm.X_i * c[2]);
+        const F e1 = Dm<F>::exp(gm.X_i * c[4]);
+        const F u = gm.X_i * c[5];
+        const F del = u * (F(1.0) + u * (F(0.5) + u * F(1.0 / 6.0)));
+        const F z2 = gm.Z_i * gm.Z_i;
+        const F sx = gm.Z_i * (c[10] - z2 * (c[11] - z2 * (c[12] - z2 * c[13]))) + c[7] * del * s2;
+        const F sy2 = gm.Z_i * (c[14] - z2 * (c[15] - z2 * (c[16] - z2 * c[17]))) + c[8] * del * s2;
+        const F sz2 = c[14] - z2 * (c[18] - z2 * (c[19] - z2 * c[20])) + c[9] * del * c2;
+        const F bx = -cy * (c[2] * e0 * sz + e1 * sx);
+        const F by = c[0] * sy * (e0 * sz + e1 * sy2);
+        const F bz = -cy * (c[1] * e0 * cz + e1 * sz2);
+        px += bx;
+        py += by * P.ct + bz * P.st;
+        pz += bz * P.ct - by * P.st;
+    }
+    // family 3: conical sheets of the ring current and the tail disk (6)
+    if (base < CO_NCONE && t < CO_NCONE) {
+        const float4 c = ld4(T.cone[t]);
+        const F f = c.x, beta = c.y, xs = gm.xx - c.z, d1 = c.w;
+        const F y20 = gm.yy / F(20.0);
+        const F dd = F(2.0) + d1 * (y20 * y20);
+        const F dzs = Dm<F>::sqrt(gm.zz * gm.zz + dd * dd);
+        const F ddz_dy = (dd * d1 * gm.yy * F(0.005)) / dzs;
+        const F ddz_dz = gm.zz / dzs;
+        const F rhos = Dm<F>::sqrt(xs * xs + gm.yy * gm.yy);
+        F drx = F(0.0), dry = (gm.yy >= 0) ? F(1.0) : F(-1.0);
+        if (rhos >= F(1.0e-5)) {
+            drx = xs / rhos;
+            dry = gm.yy / rhos;
+        }
+        F as, da1, da2, s1z, s2z, s1r, s2r;
+        cone_terms(dzs + beta, rhos, beta, &as, &da1, &da2, &s1z, &s2z, &s1r, &s2r);
+        const F dasx = da1 * (s1r * drx) + da2 * (s2r * drx);
+        const F dasy = da1 * (s1z * ddz_dy + s1r * dry) + da2 * (s2z * ddz_dy + s2r * dry);
+        const F dasz = da1 * (s1z * ddz_dz) + da2 * (s2z * ddz_dz);
+        px += -f * xs * dasz;
+        py += -f * gm.yy * dasz;
+        pz += f * ((F(2.0) * as + gm.yy * dasy) + xs * dasx);
+    }
+    // family 4: cylindrical harmonics (6)
+    if (base < CO_NCYL && t < CO_NCYL) {
+        const float4 c = ld4(T.cyl[t]);
+        const F a = c.x, scale = c.y;
+        F rho = Dm<F>::sqrt(gm.yy * gm.yy + gm.zz * gm.zz);
+        F sinfi, cosfi;
+        if (rho < F(1.0e-8)) {
+            sinfi = F(1.0);
+            cosfi = F(0.0);
+            rho = F(1.0e-8);
+        } else {
+            sinfi = gm.zz / rho;
+            cosfi = gm.yy / rho;
+        }
+        const F dz = rho / scale, xk = gm.xx / scale;
+        F j0, j1;
+        bessel01(dz, &j0, &j1);
+        const F e = Dm<F>::exp(xk);
+        if (c.z == F(0.0)) {
+            const F sinfi2 = sinfi * sinfi;
+            const F si2co2 = sinfi2 - cosfi * cosfi;
+            px += -a * j1 * e * sinfi;
+            py += a * (F(2.0) * j1 / dz - j0) * e * sinfi * cosfi;
+            pz += a * (j1 / dz * si2co2 - j0 * sinfi2) * e;
+        } else {
+            const F brho = (xk * j0 - (dz * dz + xk - F(1.0)) * j1 / dz) * e * sinfi;
+            const F bphi = (j0 + j1 / dz * (xk - F(1.0))) * e * cosfi;
+            px += a * (dz * j0 + xk * j1) * e * sinfi;
+            py += a * (brho * cosfi - bphi * sinfi);
+            pz += a * (brho * sinfi + bphi * cosfi);
+        }
+    }
+    // family 5: tail87 sheet and images (3)
+    if (base < CO_NT87 && t < CO_NT87) {
+        const float4 c = ld4(T.t87[t]);
+        const F zl = gm.zz + c.x, w = c.y;
+        const F xnx = F(-10.0) - gm.xx, xnx2 = xnx * xnx;
+        const F xc1 = gm.xx + F(1.261), xc2 = gm.xx + F(0.663);
+        const F xc22 = xc2 * xc2, xr2 = xc2 * F(-0.1071), xc12 = xc1 * xc1;
+        const F b2 = zl * zl + F(9.0);
+        const F bb = Dm<F>::sqrt(b2);
+        const F xa1 = xc12 + b2, xa2 = F(1.) / (xc22 + b2), xna = xnx2 + b2;
+        const F ff = b2 - xc22;
+        const F ln1 = Dm<F>::log(F(76.37) / xna);
+        const F ln2 = ln1 + F(0.13238005);
+        const F s0 = (Dm<F>::atan(xnx / bb) + F(1.5707963)) / bb;
+        const F s1 = (ln1 * F(.5) + xc1 * s0) / xa1;
+        const F s2 = (xc2 * xa2 * ln2 + F(0.1071) - ff * xa2 * s0) * xa2;
+        const F g1 = (b2 * s0 - F(0.5) * xc1 * ln1) / xa1;
+        const F g2 = ((F(0.5) * ff * ln2 + F(2.) * s0 * b2 * xc2) * xa2 + xr2) * xa2;
+        px += w * zl * (F(0.391734) * s0 + F(5.89715) * s1 + F(24.6833) * s2);
+        pz += w * (F(-0.5) * F(0.391734) * ln1 + F(5.89715) * g1 + F(24.6833) * g2);
+    }
+    // family 6: Region-1 dipoles of the loop basis (21) at the loop point, then of the
+    // conical basis (46) at the cone point
+    if (base < CO_NDIP_LOOP && gm.need_loop && t < CO_NDIP_LOOP) {
+        const F3 d = coop_dipole(T, t, gm.wl, gm.lx, gm.ly, gm.lz);
+        px += d.x;
+        py += d.y;
+        pz += d.z;
+    }
+    if (base < CO_NDIP - CO_NDIP_LOOP && gm.need_cone && t < CO_NDIP - CO_NDIP_LOOP) {
+        const F3 d = coop_dipole(T, CO_NDIP_LOOP + t, gm.wc, gm.cxp, gm.cyp, gm.czp);
+        px += d.x;
+        py += d.y;
+        pz += d.z;
+    }
+    // family 7: current loops (18; groups weighted by wl, wo, wi)
+    if (base < CO_NCIRC && t < CO_NCIRC) {
+        const float *c = T.circ[t];
+        const int g = (int)c[14];
+        const F w = g == 0 ? gm.wl : (g == 1 ? gm.wo : gm.wi);
+        if (w != F(0.0)) {
+            const F ux = (g == 0 ? gm.lx : gm.xx) - c[9], uy = (g == 0 ? gm.ly : gm.yy) - c[10], uz = (g == 0 ? gm.lz : gm.zz) - c[11];
+            const F qx = c[0] * ux + c[1] * uy + c[2] * uz;
+            const F qy = c[3] * ux + c[4] * uy + c[5] * uz;
+            const F qz = c[6] * ux + c[7] * uy + c[8] * uz;
+            const v3<F> bl = circle(qx, qy, qz, c[12]);
+            const F k = w * c[13];
+            px += k * (c[0] * bl.x + c[3] * bl.y + c[6] * bl.z);
+            py += k * (c[1] * bl.x + c[4] * bl.y + c[7] * bl.z);
+            pz += k * (c[2] * bl.x + c[5] * bl.y + c[8] * bl.z);
+        }
+    }
+    // family 8: conical harmonics (10: Region-1 conical basis, Region-2 inner)
+    if (base < CO_NCONIC && t < CO_NCONIC) {
+        const float4 c = ld4(T.conic[t]);
+        const bool g0 = c.w == F(0.0);
+        const F w = g0 ? gm.wc : gm.wi;
+        if (w != F(0.0)) {
+            const int m = (int)c.x;
+            const F X = (g0 ? gm.cxp : gm.xx) + c.z, Y = g0 ? gm.cyp : gm.yy, Z = g0 ? gm.czp : gm.zz;
+            // reciprocal square roots in place of the roots and divisions:
+            // 1/(r s) = 1/ro, 1/ch^2 = 1/(0.5 (1 + cos)), 1/sh^2 = 1/(0.5 (1 - cos))
+            const F ro2 = X * X + Y * Y;
+            const F irs = __builtin_amdgcn_rsqf(ro2);
+            const F ro = ro2 * irs;
+            const F cf = X * irs, sf = Y * irs;
+            const F rr2 = ro2 + Z * Z;
+            const F ir = __builtin_amdgcn_rsqf(rr2);
+            const F cc = Z * ir, sn = ro * ir;
+            const F ch2 = F(0.5) * (F(1.0) + cc), sh2 = F(0.5) * (F(1.0) - cc);
+            const F ich = __builtin_amdgcn_rsqf(ch2), ish = __builtin_amdgcn_rsqf(sh2);
+            const F tnh = sh2 * ish * ich, cnh = ch2 * ich * ish;
+            // cos/sin(m phi), tnh^m, cnh^m and the (m-1)-th powers
+            F cfm = cf, sfm = sf, tm = tnh, cm = cnh, tm1 = F(1.0), cm1 = F(1.0);
+#pragma unroll
+            for (int j = 2; j <= 5; j++) {
+                if (j <= m) {
+                    const F cn = cfm * cf - sfm * sf;
+                    sfm = sfm * cf + cfm * sf;
+                    cfm = cn;
+                    tm1 = tm;
+                    cm1 = cm;
+                    tm = tm * tnh;
+                    cm = cm * cnh;
+                }
+            }
+            const F fm = (F)m;
+            const F bt = fm * cfm * irs * (tm + cm);
+            const F bf = F(-0.5) * fm * sfm * ir * (tm1 * (ich * ich) - cm1 * (ish * ish));
+            const F k = w * c.y;
+            px += k * (bt * cc * cf - bf * sf);
+            py += k * (bt * cc * sf + bf * cf);
+            pz += k * (-bt * sn);
+        }
+    }
+    // family 9: Region-2 sheet (15)
+    if (base < CO_NSHEET && gm.ws != F(0.0) && t < CO_NSHEET) {
+        const float *c = T.sheet[t];
+        const int comp = (int)c[3];
+        F t1, t2, t3;
+        tfac(gm.xks, c[4], c[5], c[6], c[7], &t1, &t2, &t3);
+        const F rh2 = gm.xx * gm.xx + gm.yy * gm.yy;
+        const F r = Dm<F>::sqrt(rh2 + gm.zz * gm.zz);
+        const F rho = Dm<F>::sqrt(rh2);
+        const F c1p = gm.xx / rho, s1p = gm.yy / rho;
+        const F s2p = F(2.0) * s1p * c1p;
+        const F c2p = c1p * c1p - s1p * s1p;
+        const F s3p = s2p * c1p + c2p * s1p;
+        const F c3p = c2p * c1p - s2p * s1p;
+        const F s4p = s3p * c1p + c3p * s1p;
+        const F ct = gm.zz / r;
+        const F ct2 = ct * ct;
+        const F g = c[0] * (comp == 2 ? F(1.0) : ct) * Dm<F>::exp(c[1] * ct2 + c[2]);
+        const bool ycomp = comp == 1;
+        const F m0 = ycomp ? s1p : F(1.0), m1 = ycomp ? s2p : c1p, m2 = ycomp ? s3p : c2p, m3 = ycomp ? s4p : c3p;
+        const F qv = ((m0 * quad(c + 8, t1, t2, t3) + m1 * quad(c + 12, t1, t2, t3)) + m2 * quad(c + 16, t1, t2, t3)) +
+                     m3 * quad(c + 20, t1, t2, t3);
+        const F v = gm.ws * g * qv;
+        if (comp == 0) px += v;
+        else if (comp == 1) py += v;
+        else pz += v;
+    }
+    // Region-2 inner dipole distributions (one term, t = 0)
+    if (base == 0 && gm.wi != F(0.0) && t == 0) {
+        const v3<F> d6 = dipdistr(gm.xx - F(.0774), gm.yy, gm.zz, 0);
+        const v3<F> d7 = dipdistr(gm.xx + F(.038), gm.yy, gm.zz, 1);
+        const F k = gm.wi * P.b2ampl;
+        px += k * (F(29.9996) * d6.x + F(262.886) * d7.x);
+        py += k * (F(29.9996) * d6.y + F(262.886) * d7.y);
+        pz += k * (F(29.9996) * d6.z + F(262.886) * d7.z);
+    }
+    return F3{px, py, pz};
+}
+
 template <int G>
 __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, const CoopTab &T, float x, float y,
                                                    float z, int gl) {
@@ -411,10 +685,7 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
     S.q = dipole(x, y, z);
     S.fx = mk3<F>(0, 0, 0);
     if (S.sigma >= F(1.08 + 0.005)) return S;
-    CoopLane L;
-    load_lane<G>(T, gl, L);
     const F xx = x * P.xappa, yy = y * P.xappa, zz = z * P.xappa;
-    F px = 0, py = 0, pz = 0;
 
     // ---- uniform region geometry.  Region 1 (lib/t96/t96.c:619-807): which basis
     // applies, at which point, with which blend weight.
@@ -500,263 +771,20 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         }
     }
 
-    // ---- family 1: Cartesian harmonics (47 terms: one pass of 64 lanes, or three of 16
-    // over rows 0..47, row 47 being a zero row)
-    if constexpr (G == 64) {
-        const float4 h = L.harm;
-        const v3<F> b = co_harm<F>(xx, yy, zz, h.x, h.y, h.z, h.w);
-        px += b.x;
-        py += b.y;
-        pz += b.z;
-    } else {
-#pragma unroll
-        for (int k = 0; k < (CO_NHARM + G - 1) / G; k++) {
-            const float4 h = ld4(T.harm[gl + G * k]);
-            const v3<F> b = co_harm<F>(xx, yy, zz, h.x, h.y, h.z, h.w);
-            px += b.x;
-            py += b.y;
-            pz += b.z;
-        }
-    }
-    // ---- family 2: interconnection harmonics (3 lanes, one y harmonic each) at the
-    // IMF-rotated point, rotated back (lib/t96/t96.c:66-72)
-    if (gl < CO_NICON) {
-        const float *c = L.icon;
-        const F X = xx, Y = ys * P.xappa, Z = zs * P.xappa;
-        F sy, cy, sz, cz, s2, c2;
-        Dm<F>::sincos(Y * c[0], &sy, &cy);
-        Dm<F>::sincos(Z * c[1], &sz, &cz);
-        Dm<F>::sincos(Z * c[6], &s2, &c2);
-        const F e0 = c[3] * Dm<F>::exp(X * c[2]);
-        const F e1 = Dm<F>::exp(X * c[4]);
-        const F u = X * c[5];
-        const F del = u * (F(1.0) + u * (F(0.5) + u * F(1.0 / 6.0)));
-        const F z2 = Z * Z;
-        const F sx = Z * (c[10] - z2 * (c[11] - z2 * (c[12] - z2 * c[13]))) + c[7] * del * s2;
-        const F sy2 = Z * (c[14] - z2 * (c[15] - z2 * (c[16] - z2 * c[17]))) + c[8] * del * s2;
-        const F sz2 = c[14] - z2 * (c[18] - z2 * (c[19] - z2 * c[20])) + c[9] * del * c2;
-        const F bx = -cy * (c[2] * e0 * sz + e1 * sx);
-        const F by = c[0] * sy * (e0 * sz + e1 * sy2);
-        const F bz = -cy * (c[1] * e0 * cz + e1 * sz2);
-        px += bx;
-        py += by * P.ct + bz * P.st;
-        pz += bz * P.ct - by * P.st;
-    }
-    // ---- family 3: conical sheets of the ring current and the tail disk (6 lanes)
-    if (gl < CO_NCONE) {
-        const float4 c = L.cone;
-        const F f = c.x, beta = c.y, xs = xx - c.z, d1 = c.w;
-        const F y20 = yy / F(20.0);
-        const F dd = F(2.0) + d1 * (y20 * y20);
-        const F dzs = Dm<F>::sqrt(zz * zz + dd * dd);
-        const F ddz_dy = (dd * d1 * yy * F(0.005)) / dzs;
-        const F ddz_dz = zz / dzs;
-        const F rhos = Dm<F>::sqrt(xs * xs + yy * yy);
-        F drx = F(0.0), dry = (yy >= 0) ? F(1.0) : F(-1.0);
-        if (rhos >= F(1.0e-5)) {
-            drx = xs / rhos;
-            dry = yy / rhos;
-        }
-        F as, da1, da2, s1z, s2z, s1r, s2r;
-        cone_terms(dzs + beta, rhos, beta, &as, &da1, &da2, &s1z, &s2z, &s1r, &s2r);
-        const F dasx = da1 * (s1r * drx) + da2 * (s2r * drx);
-        const F dasy = da1 * (s1z * ddz_dy + s1r * dry) + da2 * (s2z * ddz_dy + s2r * dry);
-        const F dasz = da1 * (s1z * ddz_dz) + da2 * (s2z * ddz_dz);
-        px += -f * xs * dasz;
-        py += -f * yy * dasz;
-        pz += f * ((F(2.0) * as + yy * dasy) + xs * dasx);
-    }
-    // ---- family 4: cylindrical harmonics (6 lanes)
-    if (gl < CO_NCYL) {
-        const float4 c = L.cyl;
-        const F a = c.x, scale = c.y;
-        F rho = Dm<F>::sqrt(yy * yy + zz * zz);
-        F sinfi, cosfi;
-        if (rho < F(1.0e-8)) {
-            sinfi = F(1.0);
-            cosfi = F(0.0);
-            rho = F(1.0e-8);
-        } else {
-            sinfi = zz / rho;
-            cosfi = yy / rho;
-        }
-        const F dz = rho / scale, xk = xx / scale;
-        F j0, j1;
-        bessel01(dz, &j0, &j1);
-        const F e = Dm<F>::exp(xk);
-        if (c.z == F(0.0)) {
-            const F sinfi2 = sinfi * sinfi;
-            const F si2co2 = sinfi2 - cosfi * cosfi;
-            px += -a * j1 * e * sinfi;
-            py += a * (F(2.0) * j1 / dz - j0) * e * sinfi * cosfi;
-            pz += a * (j1 / dz * si2co2 - j0 * sinfi2) * e;
-        } else {
-            const F brho = (xk * j0 - (dz * dz + xk - F(1.0)) * j1 / dz) * e * sinfi;
-            const F bphi = (j0 + j1 / dz * (xk - F(1.0))) * e * cosfi;
-            px += a * (dz * j0 + xk * j1) * e * sinfi;
-            py += a * (brho * cosfi - bphi * sinfi);
-            pz += a * (brho * sinfi + bphi * cosfi);
-        }
-    }
-    // ---- family 5: tail87 sheet and images (3 lanes)
-    if (gl < CO_NT87) {
-        const float4 c = L.t87;
-        const F zl = zz + c.x, w = c.y;
-        const F xnx = F(-10.0) - xx, xnx2 = xnx * xnx;
-        const F xc1 = xx + F(1.261), xc2 = xx + F(0.663);
-        const F xc22 = xc2 * xc2, xr2 = xc2 * F(-0.1071), xc12 = xc1 * xc1;
-        const F b2 = zl * zl + F(9.0);
-        const F bb = Dm<F>::sqrt(b2);
-        const F xa1 = xc12 + b2, xa2 = F(1.) / (xc22 + b2), xna = xnx2 + b2;
-        const F ff = b2 - xc22;
-        const F ln1 = Dm<F>::log(F(76.37) / xna);
-        const F ln2 = ln1 + F(0.13238005);
-        const F s0 = (Dm<F>::atan(xnx / bb) + F(1.5707963)) / bb;
-        const F s1 = (ln1 * F(.5) + xc1 * s0) / xa1;
-        const F s2 = (xc2 * xa2 * ln2 + F(0.1071) - ff * xa2 * s0) * xa2;
-        const F g1 = (b2 * s0 - F(0.5) * xc1 * ln1) / xa1;
-        const F g2 = ((F(0.5) * ff * ln2 + F(2.) * s0 * b2 * xc2) * xa2 + xr2) * xa2;
-        px += w * zl * (F(0.391734) * s0 + F(5.89715) * s1 + F(24.6833) * s2);
-        pz += w * (F(-0.5) * F(0.391734) * ln1 + F(5.89715) * g1 + F(24.6833) * g2);
-    }
-    // ---- family 6: Region-1 dipoles (loop basis 21, conical basis 46; 1 or 2 passes)
-    {
-        const int d0 = need_loop ? 0 : CO_NDIP_LOOP;
-        const int d1 = need_cone ? CO_NDIP : CO_NDIP_LOOP;
-        for (int base = d0; base < d1; base += G) {
-            const int s = base + gl;
-            if (s < d1) {
-                const float4 dp = ld4(&T.dip[s][0]);
-                const float4 mm = ld4(&T.dip[s][4]);
-                const bool isloop = s < CO_NDIP_LOOP;
-                const F w = isloop ? wl : wc;
-                const F X = (isloop ? lx : cxp) - dp.x, Y = (isloop ? ly : cyp) - dp.y, Z = (isloop ? lz : czp) - dp.z;
-                const F r2 = X * X + Y * Y + Z * Z;
-                const F qd = w * F(30574.0) / (r2 * r2 * Dm<F>::sqrt(r2));
-                const F md = F(3.0) * (mm.x * X + mm.y * Y + mm.z * Z);
-                px += qd * (md * X - r2 * mm.x);
-                py += qd * (md * Y - r2 * mm.y);
-                pz += qd * (md * Z - r2 * mm.z);
-            }
-        }
-    }
-    // ---- family 7: current loops (18 terms; groups weighted by wl, wo, wi)
-#pragma unroll
-    for (int k = 0; k < (CO_NCIRC + G - 1) / G; k++) {
-        const int t = gl + G * k;
-        if (t >= CO_NCIRC) break;
-        float cbuf[16];
-        if constexpr (G != 64) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) put4(cbuf + 4 * j, ld4(&T.circ[t][4 * j]));
-        }
-        const float *c = G == 64 ? L.circ : cbuf;
-        const int g = (int)c[14];
-        const F w = g == 0 ? wl : (g == 1 ? wo : wi);
-        if (w != F(0.0)) {
-            const F ux = (g == 0 ? lx : xx) - c[9], uy = (g == 0 ? ly : yy) - c[10], uz = (g == 0 ? lz : zz) - c[11];
-            const F qx = c[0] * ux + c[1] * uy + c[2] * uz;
-            const F qy = c[3] * ux + c[4] * uy + c[5] * uz;
-            const F qz = c[6] * ux + c[7] * uy + c[8] * uz;
-            const v3<F> bl = circle(qx, qy, qz, c[12]);
-            const F k = w * c[13];
-            px += k * (c[0] * bl.x + c[3] * bl.y + c[6] * bl.z);
-            py += k * (c[1] * bl.x + c[4] * bl.y + c[7] * bl.z);
-            pz += k * (c[2] * bl.x + c[5] * bl.y + c[8] * bl.z);
-        }
-    }
-    // ---- family 8: conical harmonics (10 terms: Region-1 conical basis, Region-2 inner;
-    // one pass, or two with groups of 8)
-#pragma unroll
-    for (int pass = 0; pass < (CO_NCONIC + G - 1) / G; pass++) {
-        const int t = gl + G * pass;
-        if (t >= CO_NCONIC) break;
-        const float4 c = G >= 16 ? L.conic : ld4(T.conic[t]);
-        const bool g0 = c.w == F(0.0);
-        const F w = g0 ? wc : wi;
-        if (w != F(0.0)) {
-            const int m = (int)c.x;
-            const F X = (g0 ? cxp : xx) + c.z, Y = g0 ? cyp : yy, Z = g0 ? czp : zz;
-            // reciprocal square roots in place of the roots and divisions:
-            // 1/(r s) = 1/ro, 1/ch^2 = 1/(0.5 (1 + cos)), 1/sh^2 = 1/(0.5 (1 - cos))
-            const F ro2 = X * X + Y * Y;
-            const F irs = __builtin_amdgcn_rsqf(ro2);
-            const F ro = ro2 * irs;
-            const F cf = X * irs, sf = Y * irs;
-            const F rr2 = ro2 + Z * Z;
-            const F ir = __builtin_amdgcn_rsqf(rr2);
-            const F cc = Z * ir, s = ro * ir;
-            const F ch2 = F(0.5) * (F(1.0) + cc), sh2 = F(0.5) * (F(1.0) - cc);
-            const F ich = __builtin_amdgcn_rsqf(ch2), ish = __builtin_amdgcn_rsqf(sh2);
-            const F tnh = sh2 * ish * ich, cnh = ch2 * ich * ish;
-            // cos/sin(m phi), tnh^m, cnh^m and the (m-1)-th powers
-            F cfm = cf, sfm = sf, tm = tnh, cm = cnh, tm1 = F(1.0), cm1 = F(1.0);
-#pragma unroll
-            for (int j = 2; j <= 5; j++) {
-                if (j <= m) {
-                    const F cn = cfm * cf - sfm * sf;
-                    sfm = sfm * cf + cfm * sf;
-                    cfm = cn;
-                    tm1 = tm;
-                    cm1 = cm;
-                    tm = tm * tnh;
-                    cm = cm * cnh;
-                }
-            }
-            const F fm = (F)m;
-            const F bt = fm * cfm * irs * (tm + cm);
-            const F bf = F(-0.5) * fm * sfm * ir * (tm1 * (ich * ich) - cm1 * (ish * ish));
-            const F k = w * c.y;
-            px += k * (bt * cc * cf - bf * sf);
-            py += k * (bt * cc * sf + bf * cf);
-            pz += k * (-bt * s);
-        }
-    }
-    // ---- family 9: Region-2 sheet (15 terms; one pass, or two with groups of 8)
-#pragma unroll
-    for (int pass = 0; pass < (CO_NSHEET + G - 1) / G; pass++) {
-        const int t = gl + G * pass;
-        if (ws == F(0.0) || t >= CO_NSHEET) break;
-        float sbuf[24];
-        if constexpr (G < 16) {
-#pragma unroll
-            for (int jj = 0; jj < 6; jj++) put4(sbuf + 4 * jj, ld4(&T.sheet[t][4 * jj]));
-        }
-        const float *c = G >= 16 ? L.sheet : sbuf;
-        const int comp = (int)c[3];
-        F t1, t2, t3;
-        tfac(xks, c[4], c[5], c[6], c[7], &t1, &t2, &t3);
-        const F rh2 = xx * xx + yy * yy;
-        const F r = Dm<F>::sqrt(rh2 + zz * zz);
-        const F rho = Dm<F>::sqrt(rh2);
-        const F c1p = xx / rho, s1p = yy / rho;
-        const F s2p = F(2.0) * s1p * c1p;
-        const F c2p = c1p * c1p - s1p * s1p;
-        const F s3p = s2p * c1p + c2p * s1p;
-        const F c3p = c2p * c1p - s2p * s1p;
-        const F s4p = s3p * c1p + c3p * s1p;
-        const F ct = zz / r;
-        const F ct2 = ct * ct;
-        const F g = c[0] * (comp == 2 ? F(1.0) : ct) * Dm<F>::exp(c[1] * ct2 + c[2]);
-        const bool ycomp = comp == 1;
-        const F m0 = ycomp ? s1p : F(1.0), m1 = ycomp ? s2p : c1p, m2 = ycomp ? s3p : c2p, m3 = ycomp ? s4p : c3p;
-        const F qv = ((m0 * quad(c + 8, t1, t2, t3) + m1 * quad(c + 12, t1, t2, t3)) + m2 * quad(c + 16, t1, t2, t3)) +
-                     m3 * quad(c + 20, t1, t2, t3);
-        const F v = ws * g * qv;
-        if (comp == 0) px += v;
-        else if (comp == 1) py += v;
-        else pz += v;
-    }
-    // ---- Region-2 inner dipole distributions (uniform, counted on lane 0)
-    if (wi != F(0.0) && gl == 0) {
-        const v3<F> d6 = dipdistr(xx - F(.0774), yy, zz, 0);
-        const v3<F> d7 = dipdistr(xx + F(.038), yy, zz, 1);
-        const F k = wi * P.b2ampl;
-        px += k * (F(29.9996) * d6.x + F(262.886) * d7.x);
-        py += k * (F(29.9996) * d6.y + F(262.886) * d7.y);
-        pz += k * (F(29.9996) * d6.z + F(262.886) * d7.z);
-    }
-    S.fx = mk3<F>(px, py, pz);
+    const T96Geo geo = {xx, yy, zz, xx, ys * P.xappa, zs * P.xappa, lx, ly, lz, cxp, cyp, czp,
+                        wl, wc, wo, ws, wi, xks, need_loop, need_cone};
+    // ---- this lane's block of slots, pairwise in slot order
+    constexpr int NP = 64 / G;
+    constexpr int LP = ilog2c(NP);
+    const int r = lane_residue<G>(gl);
+    SlotTree<F3, NP> tree;
+    static_for<NP>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        constexpr int base = G * brevc(j, LP);
+        tree.template push<j>(coop_t96_slot(P, T, geo, base, r + base));
+    });
+    const F3 tot = tree.total();
+    S.fx = mk3<F>(tot.x, tot.y, tot.z);
     return S;
 }
 
@@ -807,11 +835,14 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
     if (nm > 13) nm = 13;
     const int kw = nm + 1;
     F am = inv_r * inv_r, pmm = F(1.), dmm = F(0.), cm = F(0.), sm = F(1.);
-    F br = F(0.), bt = F(0.), bf = F(0.);
-    constexpr int passes = GL >= 16 ? 1 : 2;      // at most 14 orders
+    // orders m = t + 1 in the canonical slots: this lane's terms t = r + GL*p; the
+    // sectoral terms advance monotonically over the passes
+    constexpr int NPI = (14 + GL - 1) / GL;   // passes that can hold an order (m <= 14)
+    const int r0 = lane_residue<GL>(gl);
+    F cbr[NPI], cbt[NPI], cbf[NPI];
 #pragma unroll
-    for (int pass = 0; pass < passes; pass++) {
-        const int m = gl + 1 + GL * pass;
+    for (int pass = 0; pass < NPI; pass++) {
+        const int m = r0 + 1 + GL * pass;
         const int m0 = pass == 0 ? 1 : m - GL;    // order the sectoral terms are at
         for (int j = 1; j < kw; j++) {
             if (j >= m0 && j < m) {
@@ -824,6 +855,7 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
                 am = am * inv_r;
             }
         }
+        F br = F(0.), bt = F(0.), bf = F(0.);
         if (m <= kw) {
             F bphi = F(0.);
             F q = pmm, dq = dmm, q2 = F(0.), dq2 = F(0.), an = am;
@@ -854,12 +886,27 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
                 hn = h2;
                 recn = rec2;
             }
-            if (m >= 2) bf = bf + bphi * (F)(m - 1);
+            if (m >= 2) bf = bphi * (F)(m - 1);
         }
+        cbr[pass] = br;
+        cbt[pass] = bt;
+        cbf[pass] = bf;
     }
-    S.br = br;
-    S.bt = bt;
-    S.bf = bf;
+    constexpr int NP = 64 / GL;
+    constexpr int LP = ilog2c(NP);
+    SlotTree<F3, NP> tree;
+    static_for<NP>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int pass = brevc(decltype(jc)::value, LP);
+        if constexpr (pass < NPI) {
+            tree.template push<decltype(jc)::value>(F3{cbr[pass], cbt[pass], cbf[pass]});
+        } else {
+            tree.template push<decltype(jc)::value>(F3{F(0.), F(0.), F(0.)});
+        }
+    });
+    const F3 tot = tree.total();
+    S.br = tot.x;
+    S.bt = tot.y;
+    S.bf = tot.z;
     return S;
 }
 

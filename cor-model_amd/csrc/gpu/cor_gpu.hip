@@ -255,21 +255,25 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
         coop->running = 0;
         coop->reparks = 0ull;
         coop->done = 0ull;
+        coop->converting = 0;
+        coop->feeders = 0;
     }
 }
 
-// Once the queue is drained, a wavefront with at most COOP_LANES_DEFAULT live
-// trajectories hands them to the cooperative mode, and a fuller one does so when
-// waiting workers times COOP_IDLE_WEIGHT reach its live count: one waiting worker starts
-// taking a wavefront's trajectories, the rest are claimed up to eight to a worker from
-// the parked pool.  Both measured on the 5-degree IGRF + T96 grid (full grid: 21.8 s with
-// 4 and 1, 17.5 s with 32 and 64 when workers held four; 16, 24, 40 and 48 lanes within
-// 0.3 s of 32 with eight).
-constexpr int COOP_LANES_DEFAULT = 32;
-constexpr long long COOP_IDLE_WEIGHT = 64;
-// After the drain, a trajectory that has run this many per-lane steps is handed over on
-// its own (0: off).
-constexpr uint32_t COOP_LONG_STEPS_DEFAULT = 0u;
+// Accepted per-lane steps before a trajectory moves to the cooperative mode.  On the
+// 5-degree IGRF + T96 grid the median trajectory runs 435 steps, 0.8 % run past 65536 and
+// hold 40 % of all steps; about 1 in 3000 reaches the 10^6-step limit.  A per-lane step
+// costs about a quarter of a cooperative one per trajectory, but a cooperative step has
+// a quarter of its latency, which the trapped trajectories need.  Full grid, measured:
+// 16.8 s (49152), 16.2 s (65536), 15.8 s (98304), 17.8 s (131072), 17.9 s (16384); 65536
+// is taken because a single scan, whose run time is the latency of its longest
+// trajectory, then leaves the per-lane mode sooner.
+constexpr uint32_t COOP_HANDOVER_DEFAULT = 65536u;
+// Parked trajectories waiting for workers, beyond eight per converting wavefront, that
+// make a feeding wavefront turn worker; and the share of wavefronts that keep feeding
+// until the queue is drained (three quarters: 16.2 s on the full grid, against 17.9 s
+// with one eighth, whose queue then drained at 10.9 s instead of 3.0 s).
+constexpr int32_t COOP_CONVERT_BACKLOG = 64;
 
 // the mixed T96 evaluator reads its interconnection series from the lane tables
 bool needs_coop_tables(int precision, int ext) { return precision != COR_PREC_FP64 && ext == EXT_T96; }
@@ -297,8 +301,9 @@ struct cor_trace_plan {
     uint64_t step_limit;
     double tu_angle, cos_tu, mass;
     int grid, block;
-    int coop_lanes;              // 0: per-lane mode only
-    uint32_t long_steps;         // per-trajectory handover after this many steps (0: off)
+    int coop_on;                 // 0: per-lane mode only
+    uint32_t handover_steps;     // per-lane accepted steps before the cooperative mode
+    size_t park_slots;
     uint32_t run_id;
     CoopCtl *d_coop;
     ParkDev *d_park;
@@ -498,11 +503,8 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     }
 
     // the cooperative evaluator covers IGRF and T96 (TS05 and SH traces stay per-lane)
-    if (p->prec == COR_PREC_MIXED && opts->tail_lanes >= 0 && ext != EXT_TS05 && cru == CRU_NONE)
-        p->coop_lanes = opts->tail_lanes == 0 ? COOP_LANES_DEFAULT : std::min(opts->tail_lanes, 64);
-    p->long_steps = 0u;
-    if (p->coop_lanes > 0 && opts->handover_steps >= 0)
-        p->long_steps = opts->handover_steps == 0 ? COOP_LONG_STEPS_DEFAULT : (uint32_t)opts->handover_steps;
+    p->coop_on = p->prec == COR_PREC_MIXED && opts->coop >= 0 && ext != EXT_TS05 && cru == CRU_NONE;
+    p->handover_steps = opts->handover_steps > 0 ? (uint32_t)opts->handover_steps : COOP_HANDOVER_DEFAULT;
 
     // persistent grid: as many resident blocks as the occupancy allows, but no more
     // blocks than are needed to give every trajectory a lane at the start (or, with the
@@ -516,24 +518,22 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     if (need * 64 < (int64_t)p->block * grid) {
         // small launches: single-wavefront blocks spread over all CUs
         p->block = 64;
-        grid = std::min<int64_t>(p->coop_lanes > 0 ? total : need, (int64_t)per_cu * 4 * g_num_cus);
+        grid = std::min<int64_t>(p->coop_on ? total : need, (int64_t)per_cu * 4 * g_num_cus);
     }
     p->grid = (int)std::max<int64_t>(grid, 1);
-    if (p->coop_lanes > 0 || needs_coop_tables(p->prec, ext)) {
+    if (p->coop_on || needs_coop_tables(p->prec, ext)) {
         // lane tables of the epoch: the cooperative evaluator's, whose T96
         // interconnection series the per-lane mixed path also reads
         PLAN_TRY(build_coop_tables(p->d_epoch, g_stream));
         PLAN_TRY(hipStreamSynchronize(g_stream));
     }
-    if (p->coop_lanes > 0) {
-        // cooperative tail: the handover counters and one park slot per lane of the
-        // grid (a lane parks at most one trajectory per run)
-        // one park slot per lane of the grid for the per-lane mode's handovers, as many
-        // again for trajectories that cooperative workers hand back
-        const size_t slots = 2 * (size_t)p->grid * p->block;
+    if (p->coop_on) {
+        // the handover counters and the park slots: every trajectory parks at most once
+        // from the per-lane mode, and workers hand back at most one per lane of the grid
+        p->park_slots = (size_t)total + (size_t)p->grid * p->block;
         PLAN_TRY(hipMalloc(&p->d_coop, sizeof(CoopCtl)));
-        PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * slots));
-        PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * slots, g_stream));
+        PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * p->park_slots));
+        PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * p->park_slots, g_stream));
         PLAN_TRY(hipStreamSynchronize(g_stream));
     }
     for (int k = 0; k < 3; k++) PLAN_TRY(hipEventCreate(&p->ev[k]));
@@ -562,7 +562,7 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.mass = p->mass;
     a.inv_mass = 1.0 / p->mass;
     a.out = p->out;
-    a.coop_lanes = p->coop_lanes;
+    a.coop_on = p->coop_on;
     a.run_id = ++p->run_id;      // park records of earlier runs never match
     a.coop = p->d_coop;
     a.park = p->d_park;
@@ -570,8 +570,15 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.repark_limit = (long long)p->grid * p->block;
     a.trk = p->d_trk;
     a.trk_off = p->d_trk_off;
-    a.idle_weight = COOP_IDLE_WEIGHT;
-    a.long_steps = p->long_steps;
+    a.handover_steps = p->handover_steps;
+    a.convert_backlog = COOP_CONVERT_BACKLOG;
+    a.min_feeders = std::max(1, (int)((int64_t)p->grid * (p->block / 64) * 3 / 4));
+    {
+        // longer than any trajectory can run: 60 s + 1 ms per allowed step
+        const double secs = 60.0 + 1e-3 * (double)p->step_limit;
+        const double ticks = secs * (g_wall_khz > 0. ? g_wall_khz * 1e3 : 1e8);
+        a.idle_guard = ticks < 1e18 ? (unsigned long long)ticks : 1000000000000000000ull;
+    }
     reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop, p->d_run);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(p->ev[0], s));
@@ -597,7 +604,7 @@ int cor_trace_plan_sync(cor_trace_plan *p) {
 int cor_trace_plan_results(cor_trace_plan *p, cor_traj *traj, cor_cutoff *cut) {
     if (!p || !p->ran) return COR_E_INVALID;
     HIP_TRY(hipEventSynchronize(p->ev[2]));
-    if (p->coop_lanes > 0) {
+    if (p->coop_on) {
         // every trajectory is counted once when it ends, in either mode
         CoopCtl c;
         HIP_TRY(hipMemcpy(&c, p->d_coop, sizeof c, hipMemcpyDeviceToHost));
@@ -692,7 +699,7 @@ int cor_trace_plan_set_epoch(cor_trace_plan *p, const cor_model_set *models, con
     build_epoch(models, epoch, *p->h_epoch);
     HIP_TRY(upload_sh(models, &p->d_sh, &p->sh_cap, *p->h_epoch));
     HIP_TRY(hipMemcpyAsync(p->d_epoch, p->h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s));
-    if (p->coop_lanes > 0 || needs_coop_tables(p->prec, p->ext)) HIP_TRY(build_coop_tables(p->d_epoch, s));
+    if (p->coop_on || needs_coop_tables(p->prec, p->ext)) HIP_TRY(build_coop_tables(p->d_epoch, s));
     HIP_TRY(hipEventRecord(p->ev_up, s));
     p->up_pending = true;
     return COR_OK;

@@ -104,6 +104,8 @@ struct CoopCtl {
     alignas(128) int32_t running;             // wavefronts still in the per-lane mode
     alignas(128) unsigned long long reparks;  // trajectories handed back by cooperative workers
     alignas(128) unsigned long long done;     // trajectories finished (both modes)
+    alignas(128) int32_t converting;          // wavefronts running down their lanes to turn worker
+    alignas(128) int32_t feeders;             // wavefronts taking new trajectories from the queue
 };
 
 struct ScanDev {
@@ -150,10 +152,10 @@ struct TraceArgs {
     double mass;
     double inv_mass;             // 1 / mass
     TraceOut out;
-    // cooperative tail (mixed precision): once the queue is drained, a wavefront with
-    // at most coop_lanes live trajectories parks them and turns into a worker that runs
-    // parked trajectories one per wavefront (0: per-lane mode only)
-    int32_t coop_lanes;
+    // cooperative mode (mixed precision, IGRF/T96 model sets): a trajectory that has
+    // run handover_steps accepted steps in the per-lane mode is parked and continued by
+    // cooperative workers, a group of lanes per trajectory (coop_on = 0: per-lane only)
+    int32_t coop_on;
     uint32_t run_id;
     CoopCtl *coop;
     ParkDev *park;
@@ -161,9 +163,15 @@ struct TraceArgs {
     long long repark_limit;      // park slots beyond the per-lane ones (one per lane of the grid)
     cor_track_point *trk;        // FP64 track recording (NULL: off)
     const int64_t *trk_off;      // first point of each trajectory (n_items + 1 entries)
-    long long idle_weight;       // trajectories one waiting worker is counted to take
-    uint32_t long_steps;         // per-lane steps after which a trajectory is handed over
-                                 // alone once the queue is drained (0: off)
+    uint32_t handover_steps;     // per-lane accepted steps before the handover
+    int32_t convert_backlog;     // parked trajectories waiting, beyond what converting
+                                 // wavefronts will take, that turn a feeding wavefront
+                                 // into a worker
+    int32_t min_feeders;         // wavefronts that keep taking new trajectories until the
+                                 // queue is drained (so it always drains)
+    unsigned long long idle_guard;   // wall-clock ticks without a finished trajectory after
+                                     // which an idle worker gives up (a lost-record guard
+                                     // longer than any trajectory can legitimately run)
 };
 
 }  // namespace cor

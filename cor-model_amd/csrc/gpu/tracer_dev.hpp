@@ -614,7 +614,9 @@ __device__ __forceinline__ v3<float> coop_field_spread(const EpochView<float> &E
     return mk3(__shfl(fo.x, from), __shfl(fo.y, from), __shfl(fo.z, from));
 }
 
-// Cooperative worker.  A wavefront runs up to eight parked trajectories, one per group of
+// Cooperative worker; returns true when the run is over, false when the wavefront should
+// go back to the per-lane mode (it is idle, no parked trajectory waits and the queue still
+// has new ones).  A wavefront runs up to eight parked trajectories, one per group of
 // COOP_GROUP lanes, while parked work is plentiful; with four or fewer the field is
 // evaluated with wider groups (a lone trajectory with all 64 lanes: its step then takes
 // about two thirds of the time).  Groups of a wavefront hold the same state in all
@@ -629,13 +631,14 @@ __device__ __forceinline__ v3<float> coop_field_spread(const EpochView<float> &E
 //     longest-running first.
 // An idle wavefront keeps its ticket until every trajectory of the run has finished
 // (a counter both modes add to), so a trajectory handed back is always taken; then it
-// retires.  Every slot is served by the holder of its ticket, so the count is reached;
-// as a guard against a lost record, an idle wavefront also retires after 60 s without
-// any trajectory finishing once no wavefront is left in the per-lane mode (the host
-// then reports the shortfall).  Workers never wait for one another and parkers never
-// wait.
+// retires.  Every slot is served by the holder of its ticket, so the count is reached.
+// The only other exit is a guard against a lost record: with no wavefront left in the
+// per-lane mode and no trajectory finished for idle_guard ticks -- set longer than one
+// trajectory can run to the step limit at the slowest step of any mode, so it cannot
+// fire while a trajectory is still running -- an idle wavefront retires and the host
+// reports the shortfall.  Workers never wait for one another and parkers never wait.
 template <int INT, int EXT>
-__device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const CoopTab &T, int lane) {
+__device__ bool coop_worker(const TraceArgs &a, const EpochView<float> &E, const CoopTab &T, int lane) {
     constexpr int GS = COOP_GROUP;
     const int gl = lane & (GS - 1);
     const int lead = lane & ~(GS - 1);            // first lane of this lane's group
@@ -669,6 +672,9 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
             int want = 0;
             bool wait_ok = false;
             if (!has_leads && !__any(holding)) {
+                // idle: with no parked trajectory waiting and the queue not yet drained,
+                // this wavefront goes back to taking new trajectories in the per-lane mode
+                if (head0 >= res0 && load_relaxed_uniform(a.head) < (unsigned long long)a.n_items) return false;
                 want = 1;
                 wait_ok = true;
             } else if (free_leads && head0 < res0) {
@@ -724,27 +730,26 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
             has_leads = __ballot(has && is_lead);
             if (!has_leads) {
                 const unsigned long long done = load_relaxed_uniform(&a.coop->done);
-                if (done >= (unsigned long long)a.n_items) return;
+                if (done >= (unsigned long long)a.n_items) return true;
                 const unsigned long long now = wall_clock64();
                 if (done != done_seen || running != 0) {
                     done_seen = done;
                     idle_since = now;
-                } else if (now - idle_since > 60ull * 100000000ull) {
-                    return;
+                } else if (now - idle_since > a.idle_guard) {
+                    return true;
                 }
                 __builtin_amdgcn_s_sleep(127);
                 continue;
             }
         }
         // ---- 2. spread: with two or more trajectories here and a wavefront waiting for
-        // work, hand the last group's trajectory back
+        // work, hand back the one with the most steps so far (most likely the longest to
+        // go), which then runs alone
         if (look && __popcll(has_leads) >= 2 && repark_ok) {
             const unsigned long long head1 = load_relaxed_uniform(&a.coop->head);
             const unsigned long long res1 = load_relaxed_uniform(&a.coop->res);
+            int src = -1;
             if (head1 > res1) {
-                // the trajectory with the most steps so far (most likely the longest
-                // to go) moves to the waiting wavefront, where it runs alone
-                int src = -1;
                 unsigned best = 0;
                 for (uint64_t m = has_leads; m; m &= m - 1) {
                     const int l = __ffsll((unsigned long long)m) - 1;
@@ -754,6 +759,8 @@ __device__ void coop_worker(const TraceArgs &a, const EpochView<float> &E, const
                         src = l;
                     }
                 }
+            }
+            if (src >= 0) {
                 unsigned long long r = 0;
                 if (lane == src) r = atomicAdd(&a.coop->reparks, 1ull);
                 r = __shfl(r, src);
@@ -841,7 +848,7 @@ trace_kernel(TraceArgs a) {
     constexpr bool kCoop = PREC == PREC_MIXED && EXT != EXT_TS05 && CRU == CRU_NONE;   // the cooperative evaluator's models
     if constexpr (kCoop) {
         __shared__ CoopTab s_coop;
-        if (a.coop_lanes > 0) {
+        if (a.coop_on) {
             stage_coop(&a.epoch->coop, &s_coop);
             if (lane == 0) atomicAdd(&a.coop->running, 1);
         }
@@ -849,7 +856,9 @@ trace_kernel(TraceArgs a) {
     }
 
     bool active = false;
-    bool queue_open = true;
+    // this wavefront refills ended lanes from the queue (until the queue is drained, or
+    // it turns into a cooperative worker)
+    bool feeding, converting;
     int64_t gid = 0;
     int scan = 0;
     v3<double> pos = mk3(0., 0., 0.), vel = mk3(0., 0., 0.);
@@ -858,147 +867,177 @@ trace_kernel(TraceArgs a) {
     int nk0 = 0, nk1 = 0;
     unsigned int step = 0;
     uint32_t attempts = 0;
-    // queue head as read one step earlier: a wavefront whose lanes are all live never
-    // refills, so it learns that other wavefronts drained the queue from this
-    unsigned long long head_seen = 0;
-    // cooperative workers waiting for work (tickets taken minus slots reserved), as
-    // read one step earlier
-    long long idle_seen = 0;
+    unsigned iter = 0;
     // fill of this wavefront's per-lane loop
     unsigned long long wave_iters = 0, lane_iters = 0, wave_iters_open = 0, lane_iters_open = 0;
     unsigned long long ended_here = 0;
     if (lane == 0) atomicMin(&a.stats->t_start, (unsigned long long)wall_clock64());
-
-    for (;;) {
-        // ---- 1. refill ended lanes from the queue
-        if (queue_open) {
-            const uint64_t idle = __ballot(!active);
-            if (idle) {
-                const int leader = __ffsll((unsigned long long)idle) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(a.head, (unsigned long long)__popcll(idle));
-                base = __shfl(base, leader);
-                if (base + __popcll(idle) >= (unsigned long long)a.n_items) {
-                    queue_open = false;
-                    // this wavefront took the queue's last slot
-                    if (lane == leader && base < (unsigned long long)a.n_items)
-                        a.stats->t_drain = (unsigned long long)wall_clock64();
-                }
-                if (!active) {
-                    const unsigned long long slot = base + __popcll(idle & ((1ull << lane) - 1ull));
-                    if (slot < (unsigned long long)a.n_items) {
-                        gid = a.order ? (int64_t)a.order[slot] + a.id_base : (int64_t)slot + a.id_base;
-                        scan = find_scan(a.scans, a.n_scans, gid);
-                        const ScanDev &S = a.scans[scan];
-                        const int i = (int)(gid - S.first);
-                        // src/trajectory.c:13-43
-                        zn = S.zn;
-                        nk1 = S.nk1;
-                        nk0 = nk1;
-                        const double rig = S.rig + i * S.del;
-                        const double aa = rig * 1.0e+09 * zn * K_Q / (a.mass * K_C * K_C);
-                        v = K_C * sqrt(1. - 1. / (1. + aa * aa));
-                        const double rs = v * S.st1;
-                        vel = mk3(rs * S.cp1, rs * S.sp1, v * S.ct1);
-                        pos = mk3(S.pos0[0], S.pos0[1], S.pos0[2]);
-                        time = 0.;
-                        alength = 0.;
-                        step = 0;
-                        attempts = 0;
-                        active = true;
-                    }
-                }
+    // per-lane phase, then (mixed, cooperative model sets) the cooperative worker, which
+    // may hand the wavefront back to the per-lane mode while the queue has trajectories
+    for (int phase = 0;; phase++) {
+        feeding = true;
+        converting = false;
+        ended_here = 0;
+        if constexpr (kCoop) {
+            if (a.coop_on && lane == 0) {
+                if (phase > 0) atomicAdd(&a.coop->running, 1);
+                atomicAdd(&a.coop->feeders, 1);
             }
         }
-        if constexpr (kCoop) {
-            // queue drained: a sparse wavefront hands its trajectories to the
-            // cooperative mode and becomes a worker itself
-            // and parks when its live trajectories are few, or when enough workers are
-            // idle to take every one of them (a worker's step is ~5x shorter than a
-            // per-lane step, so idle workers are always the faster way)
-            if (a.coop_lanes > 0) {
-                if (queue_open && head_seen >= (unsigned long long)a.n_items) queue_open = false;
-                if (!queue_open) {
-                    const int live = __popcll(__ballot(active));
-                    if (live <= a.coop_lanes || idle_seen * a.idle_weight >= (long long)live) {
-                        park_lanes(a, lane, active, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
-                        break;
+
+        for (;;) {
+            iter++;
+            // ---- 1. refill ended lanes from the queue
+            if (feeding) {
+                const uint64_t idle = __ballot(!active);
+                if (idle) {
+                    const int leader = __ffsll((unsigned long long)idle) - 1;
+                    unsigned long long base = 0;
+                    if (lane == leader) base = atomicAdd(a.head, (unsigned long long)__popcll(idle));
+                    base = __shfl(base, leader);
+                    if (base + __popcll(idle) >= (unsigned long long)a.n_items) {
+                        feeding = false;
+                        if constexpr (kCoop) {
+                            if (a.coop_on && lane == leader) atomicSub(&a.coop->feeders, 1);
+                        }
+                        // this wavefront took the queue's last slot
+                        if (lane == leader && base < (unsigned long long)a.n_items)
+                            a.stats->t_drain = (unsigned long long)wall_clock64();
                     }
-                    // a long-running trajectory leaves a full wavefront on its own: at
-                    // a per-lane step's latency it would be the run's last to finish
-                    if (a.long_steps) {
-                        const bool lng = active && step >= a.long_steps;
-                        if (__ballot(lng)) {
-                            park_lanes(a, lane, lng, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
-                            if (lng) active = false;
+                    if (!active) {
+                        const unsigned long long slot = base + __popcll(idle & ((1ull << lane) - 1ull));
+                        if (slot < (unsigned long long)a.n_items) {
+                            gid = a.order ? (int64_t)a.order[slot] + a.id_base : (int64_t)slot + a.id_base;
+                            scan = find_scan(a.scans, a.n_scans, gid);
+                            const ScanDev &S = a.scans[scan];
+                            const int i = (int)(gid - S.first);
+                            // src/trajectory.c:13-43
+                            zn = S.zn;
+                            nk1 = S.nk1;
+                            nk0 = nk1;
+                            const double rig = S.rig + i * S.del;
+                            const double aa = rig * 1.0e+09 * zn * K_Q / (a.mass * K_C * K_C);
+                            v = K_C * sqrt(1. - 1. / (1. + aa * aa));
+                            const double rs = v * S.st1;
+                            vel = mk3(rs * S.cp1, rs * S.sp1, v * S.ct1);
+                            pos = mk3(S.pos0[0], S.pos0[1], S.pos0[2]);
+                            time = 0.;
+                            alength = 0.;
+                            step = 0;
+                            attempts = 0;
+                            active = true;
                         }
                     }
-                    idle_seen = (long long)(__hip_atomic_load(&a.coop->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
-                                            __hip_atomic_load(&a.coop->res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                } else {
-                    head_seen = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
-        }
-        const uint64_t live_mask = __ballot(active);
-        if (!live_mask) break;
-        wave_iters++;
-        lane_iters += __popcll(live_mask);
-        if (queue_open) {
-            wave_iters_open++;
-            lane_iters_open += __popcll(live_mask);
-        }
-
-        bool ended_now = false;
-        if (active) {
-            // ---- 2. field at the current point (src/trajectory.c:37-38,147-148); the
-            // GSM position is recomputed from the GEO state (same arithmetic as the
-            // reference's geo_to_gsm after an accepted step)
-            const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
-            if constexpr (PREC == PREC_FP64) {
-                const v3<double> f = field_fp64<INT, EXT, CRU>(E, gsm);
-                fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
-                b = 1.E-09 * sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
-            } else {
-                const v3<float> f = field_mixed<INT, EXT, CRU>(E, pos, gsm);
-                fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
-                b = inv_b_mixed(f);   // the mixed step takes 1/b
+            if constexpr (kCoop) {
+                if (a.coop_on) {
+                    // ---- 2. handover: a trajectory that has run handover_steps accepted steps
+                    // here continues in the cooperative mode.  The step count alone decides,
+                    // so each trajectory's arithmetic (per-lane up to that step, cooperative
+                    // after it, whose sums have one order for every group width) is the
+                    // same in every run, whatever the scheduling.
+                    const bool ho = active && step >= a.handover_steps;
+                    if (__ballot(ho)) {
+                        park_lanes(a, lane, ho, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
+                        if (ho) active = false;
+                    }
+                    // ---- 3. a feeding wavefront stops taking new trajectories when parked ones
+                    // wait for workers beyond what converting wavefronts will take (eight each),
+                    // as long as min_feeders others still feed; it runs its lanes down (each at
+                    // most handover_steps steps) and turns worker
+                    if (feeding && (iter & 15u) == 0) {
+                        const long long backlog =
+                            (long long)(load_relaxed_uniform(&a.coop->res) - load_relaxed_uniform(&a.coop->head));
+                        if (backlog > (long long)a.convert_backlog) {
+                            int c = 0;
+                            if (lane == 0) {
+                                // leave the feeders only above their floor (decrement first, so
+                                // concurrent conversions cannot take the count below it)
+                                const int f = atomicSub(&a.coop->feeders, 1);
+                                c = atomicAdd(&a.coop->converting, 1);
+                                if (f <= a.min_feeders || backlog <= 8ll * c + a.convert_backlog) {
+                                    atomicSub(&a.coop->converting, 1);
+                                    atomicAdd(&a.coop->feeders, 1);
+                                    c = -1;
+                                }
+                            }
+                            c = __shfl(c, 0);
+                            if (c >= 0) {
+                                feeding = false;
+                                converting = true;
+                            }
+                        }
+                    }
+                }
+            }
+            const uint64_t live_mask = __ballot(active);
+            if (!live_mask) {
+                if (feeding) continue;   // every lane was just handed over: refill
+                break;
+            }
+            wave_iters++;
+            lane_iters += __popcll(live_mask);
+            if (feeding) {
+                wave_iters_open++;
+                lane_iters_open += __popcll(live_mask);
             }
 
-            // ---- 3. attempts until one is accepted or the trajectory ends
-            int status;
-            if constexpr (PREC == PREC_FP64) {
-                status = attempts_fp64(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
-            } else {
-                status = attempts_mixed(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
+            bool ended_now = false;
+            if (active) {
+                // ---- 4. field at the current point (src/trajectory.c:37-38,147-148); the
+                // GSM position is recomputed from the GEO state (same arithmetic as the
+                // reference's geo_to_gsm after an accepted step)
+                const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
+                if constexpr (PREC == PREC_FP64) {
+                    const v3<double> f = field_fp64<INT, EXT, CRU>(E, gsm);
+                    fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
+                    b = 1.E-09 * sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
+                } else {
+                    const v3<float> f = field_mixed<INT, EXT, CRU>(E, pos, gsm);
+                    fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
+                    b = inv_b_mixed(f);   // the mixed step takes 1/b
+                }
+
+                // ---- 5. attempts until one is accepted or the trajectory ends
+                int status;
+                if constexpr (PREC == PREC_FP64) {
+                    status = attempts_fp64(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
+                } else {
+                    status = attempts_mixed(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
+                }
+                if (status >= 0) {
+                    write_end(a, gid, status, step, attempts);
+                    active = false;
+                    ended_now = true;
+                } else if constexpr (PREC == PREC_FP64) {
+                    if (a.trk) write_track(a, gid, step, E.A, pos, v, time, alength);
+                }
             }
-            if (status >= 0) {
-                write_end(a, gid, status, step, attempts);
-                active = false;
-                ended_now = true;
-            } else if constexpr (PREC == PREC_FP64) {
-                if (a.trk) write_track(a, gid, step, E.A, pos, v, time, alength);
+            // trajectories ended here, added to the run's count when the wavefront leaves
+            // the per-lane mode
+            ended_here += __popcll(__ballot(ended_now));
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats->wave_iters, wave_iters);
+            atomicAdd(&a.stats->lane_iters, lane_iters);
+            atomicAdd(&a.stats->wave_iters_open, wave_iters_open);
+            atomicAdd(&a.stats->lane_iters_open, lane_iters_open);
+            atomicMax(&a.stats->t_lane_end, (unsigned long long)wall_clock64());
+        }
+        wave_iters = lane_iters = wave_iters_open = lane_iters_open = 0;
+        if constexpr (kCoop) {
+            if (a.coop_on) {
+                if (lane == 0) {
+                    if (ended_here) atomicAdd(&a.coop->done, ended_here);
+                    if (converting) atomicSub(&a.coop->converting, 1);
+                    __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (coop_worker<INT, EXT>(a, E, *ctab, lane)) break;
+                continue;
             }
         }
-        // trajectories ended here, added to the run's count when the wavefront leaves
-        // the per-lane mode
-        ended_here += __popcll(__ballot(ended_now));
-    }
-    if (lane == 0) {
-        atomicAdd(&a.stats->wave_iters, wave_iters);
-        atomicAdd(&a.stats->lane_iters, lane_iters);
-        atomicAdd(&a.stats->wave_iters_open, wave_iters_open);
-        atomicAdd(&a.stats->lane_iters_open, lane_iters_open);
-        atomicMax(&a.stats->t_lane_end, (unsigned long long)wall_clock64());
-    }
-    if constexpr (kCoop) {
-        if (a.coop_lanes > 0) {
-            if (lane == 0) {
-                if (ended_here) atomicAdd(&a.coop->done, ended_here);
-                __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            coop_worker<INT, EXT>(a, E, *ctab, lane);
-        }
+        break;
     }
 }
 

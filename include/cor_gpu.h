@@ -77,15 +77,13 @@ typedef struct {
     double particle_mass;     /* DEF_HM0_P or DEF_HM0_E */
     int32_t precision;        /* COR_PREC_FP64 or COR_PREC_MIXED */
     int32_t queue_order;      /* 0 = rigidity order, 1 = longest-expected first */
-    int32_t tail_lanes;       /* COR_PREC_MIXED only: once the queue is drained, a wavefront
-                               * with at most this many live trajectories hands them to the
-                               * cooperative mode (a group of lanes per trajectory); 0 = the
-                               * library default (32), negative = per-lane mode only; fuller
-                               * wavefronts also hand over when a worker waits for work */
-    int32_t handover_steps;   /* with the cooperative tail: once the queue is drained, a
-                               * trajectory that has run this many steps in the per-lane
-                               * mode is handed over on its own, however full its
-                               * wavefront; 0 = the library default, negative = off */
+    int32_t coop;             /* COR_PREC_MIXED with IGRF and/or T96: 0 = the cooperative mode
+                               * (the library default), negative = per-lane mode only */
+    int32_t handover_steps;   /* with the cooperative mode: accepted steps a trajectory runs in
+                               * the per-lane mode (one lane each) before it continues in the
+                               * cooperative mode (a group of lanes each); 0 = the library
+                               * default (65536).  A fixed step count: the results do not
+                               * depend on scheduling */
 } cor_trace_opts;
 
 /* Per trajectory.  The doubles are set only for allowed trajectories: exit v/c, exit

@@ -55,7 +55,7 @@ def test_oblique_directions_against_oracle(cor, oracle, precision):
     rows = direction_rows(cor)
     p, po = params(cor, oracle)
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
-    tg, cg = cor.trace(ms, p, rows, step_limit=200000, precision=precision, tail_lanes=-1)
+    tg, cg = cor.trace(ms, p, rows, step_limit=200000, precision=precision, coop=-1)
     off = 0
     for k, row in enumerate(rows):
         n = cor.scan_count(row)

@@ -105,6 +105,25 @@ def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts, lanes
     assert np.quantile(err, 0.99) < 1e-4
 
 
+@pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
+def test_cooperative_field_same_bits_in_every_group_width(cor, name, kinds, parmod, pts):
+    """The cooperative evaluator sums its terms in one canonical order (term t in slot
+    brev6(t) of 64, slots summed pairwise in slot order) whatever the number of lanes
+    per point, so 64, 32, 16 and 8 lanes give bit-identical fields -- the property that
+    makes a trajectory's results independent of how many trajectories shared its
+    wavefront.  The points include the magnetopause blend, the Region-1 band edges and
+    the Region-2 xi bands (the box spans the T96 domain)."""
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
+    ms = cor.model_set(**kinds)
+    outs = {lanes: cor.field(ms, p, pts, prec) for lanes, prec in
+            ((64, cor.PREC_MIXED_WAVE), (32, cor.PREC_MIXED_GROUP32), (16, cor.PREC_MIXED_GROUP16),
+             (8, cor.PREC_MIXED_GROUP))}
+    for lanes in (32, 16, 8):
+        same = outs[lanes] == outs[64]
+        print(name, lanes, "lanes: identical", same.all(axis=1).mean())
+        assert same.all(), np.argwhere(~same.all(axis=1))[:5].ravel()
+
+
 TS05_CASES = [
     ("ts05", dict(internal=0, external=2), [2, -20, 0, -2, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], box_points(6, 4000)),
     ("igrf+ts05 storm", dict(internal=1, external=2), [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5],

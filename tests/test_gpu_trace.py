@@ -106,17 +106,18 @@ def test_step_limit_and_nan_classification(cor, oracle):
     assert abs(cg["rc"] - cc[2]) < 1e-12 and abs(cg["ru"] - cc[1]) < 1e-12
 
 
-@pytest.mark.parametrize("tail_lanes", [-1, 0, 64], ids=["per-lane", "default-tail", "all-cooperative"])
-def test_mixed_config1_close(cor, oracle, tail_lanes):
+@pytest.mark.parametrize("coop,handover", [(-1, 0), (0, 0), (0, 1)],
+                         ids=["per-lane", "default", "all-cooperative"])
+def test_mixed_config1_close(cor, oracle, coop, handover):
     """Mixed precision on config 1: the effective cutoff within 0.05 GV of the FP64
     oracle, flags identical away from the (union) penumbra band, exit and asymptotic
     directions of the allowed trajectories there within 0.5 degree -- in the per-lane
-    mode only, with the default cooperative tail, and with every trajectory handed to
-    the cooperative mode once the queue is drained."""
+    mode only, with the default handover, and with every trajectory in the cooperative
+    mode from its first step."""
     osc = scan(46.808601777, 8.0)
-    row, tg, cg, tc, cc = run_both(cor, oracle, osc, dict(internal=1, external=0), precision=1,
-                                   tail_lanes=tail_lanes)
-    print("mixed", tail_lanes, cg, "fp64 cpu", cc)
+    row, tg, cg, tc, cc = run_both(cor, oracle, osc, dict(internal=1, external=0), precision=1, coop=coop,
+                                   handover_steps=handover)
+    print("mixed", coop, handover, cg, "fp64 cpu", cc)
     compare(row, tg, cg, tc, cc, rc_tol=0.05, dir_tol=0.5, dir_margin=1.0, exit_tol=5.0)
 
 
@@ -125,7 +126,7 @@ def test_t96_sites_cooperative(cor, oracle, lat, lon):
     """IGRF + T96 with every trajectory in the cooperative mode, against the FP64 oracle."""
     osc = scan(lat, lon, rig=0.5, dl=0.05, rk=20.0)
     row, tg, cg, tc, cc = run_both(cor, oracle, osc, dict(internal=1, external=1), parmod=[2, -20, 0, -2],
-                                   precision=1, tail_lanes=64)
+                                   precision=1, handover_steps=1)
     print(lat, lon, "cooperative", cg, "cpu", cc)
     # reduced precision moves individual flags only inside the chaotic band, so Rc can
     # move by up to the band's width; half of it is the bar (at least two steps)
@@ -135,30 +136,58 @@ def test_t96_sites_cooperative(cor, oracle, lat, lon):
 
 def test_cooperative_handover_many_scans(cor):
     """Many scans through the handover: every trajectory finishes exactly once (status
-    set, steps counted) whichever mode ran it, and the per-scan cutoffs of the per-lane
-    and cooperative runs agree within the penumbra noise.  Cases: per-lane only; sparse
-    wavefronts handed over (8 lanes); every wavefront at the drain (64); and single
-    trajectories leaving full wavefronts after 2 000 per-lane steps."""
+    set, steps counted) whichever mode ran it, a handed-over trajectory left the
+    per-lane mode exactly at the handover step, and the per-scan cutoffs of the per-lane
+    and cooperative runs agree within the penumbra noise.  Cases: per-lane only; the
+    default handover; handover after 2 000 and after 50 steps."""
     sites = cor.grid_sites()[::97]
     rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
     out = {}
-    cases = [(-1, -1), (8, -1), (64, -1), (8, 2000)]
-    for tl, hs in cases:
-        tr, cut = cor.trace(ms, p, rows, step_limit=200000, precision=cor.PREC_MIXED, tail_lanes=tl,
-                            handover_steps=hs)
+    cases = [(-1, 0), (0, 0), (0, 2000), (0, 50)]
+    for coop, hs in cases:
+        tr, cut = cor.trace(ms, p, rows, step_limit=200000, precision=cor.PREC_MIXED, coop=coop, handover_steps=hs)
         assert np.all(np.isin(tr["status"], (0, 1, 2))) and np.all(tr["steps"] >= 1)
-        if hs > 0:
-            # a trajectory handed over on its own left at its 2 000th step or later
-            ho = tr["handover"][tr["handover"] > 0]
-            print("handed over", len(ho), "min handover step", ho.min() if len(ho) else None)
-            assert len(ho) > 0
-        out[(tl, hs)] = cut
+        ho = tr["handover"][tr["handover"] > 0]
+        if coop < 0:
+            assert len(ho) == 0
+        elif hs > 0:
+            # handed over exactly after hs per-lane steps (recorded as 1 + that step)
+            print("handed over", len(ho), "at", np.unique(ho))
+            assert len(ho) > 0 and np.all(ho == hs + 1)
+            assert np.all(tr["steps"][tr["handover"] > 0] > hs)
+        out[(coop, hs)] = cut
     for key in cases[1:]:
-        d = np.abs(out[key]["rc"] - out[(-1, -1)]["rc"])
+        d = np.abs(out[key]["rc"] - out[(-1, 0)]["rc"])
         print(key, "max |dRc|", d.max(), "frac <= 0.1 GV", (d <= 0.1 + 1e-9).mean())
         assert (d <= 0.1 + 1e-9).mean() >= 0.8
+
+
+def test_mixed_default_is_deterministic(cor):
+    """The default mixed path (per-lane up to the handover step, then the cooperative
+    mode with one summation order for every group width) gives bit-identical records and
+    cutoffs in repeated runs and in both queue orders, over a few hundred grid scans:
+    scheduling decides only when a trajectory runs, not what it computes
+    (the reference computes each trajectory sequentially in one thread,
+    src/trajectory.c:12-153).  A short handover puts most long trajectories through
+    the cooperative mode."""
+    sites = cor.grid_sites()[::9]
+    rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
+    ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
+    runs = [(0, 1), (0, 1), (0, 0), (300, 1), (300, 0), (300, 1)]
+    ref = {}
+    for hs, q in runs:
+        tr, cut = cor.trace(ms, p, rows, step_limit=100000, precision=cor.PREC_MIXED, handover_steps=hs,
+                            queue_order=q)
+        if hs not in ref:
+            ref[hs] = (tr, cut)
+            print(len(rows), "scans, handover", hs, ":", int((tr["handover"] > 0).sum()), "trajectories cooperative")
+            continue
+        t0, c0 = ref[hs]
+        assert np.array_equal(tr, t0), (hs, q, int((tr != t0).sum()))
+        assert np.array_equal(cut, c0), (hs, q)
 
 
 TS05_STORM = [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5]
@@ -205,8 +234,8 @@ def test_sh_ts05_site_fp64(cor, oracle):
     compare(row[0], tg, cg[0], tc, cc, rc_tol=0.25)
 
 
-@pytest.mark.parametrize("precision,tail", [(0, 0), (1, -1)], ids=["fp64", "mixed-per-lane"])
-def test_queue_order_leaves_results_unchanged(cor, precision, tail):
+@pytest.mark.parametrize("precision,coop", [(0, 0), (1, -1)], ids=["fp64", "mixed-per-lane"])
+def test_queue_order_leaves_results_unchanged(cor, precision, coop):
     """The queue order only decides when a trajectory runs, not what it computes: FP64 and
     the per-lane mixed path give bit-identical per-trajectory records and cutoffs in
     rigidity order (0) and longest-expected-first order (1, the benchmark's)."""
@@ -214,7 +243,7 @@ def test_queue_order_leaves_results_unchanged(cor, precision, tail):
     rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
-    out = [cor.trace(ms, p, rows, step_limit=100000, precision=precision, tail_lanes=tail, queue_order=q)
+    out = [cor.trace(ms, p, rows, step_limit=100000, precision=precision, coop=coop, queue_order=q)
            for q in (0, 1)]
     (t0, c0), (t1, c1) = out
     assert np.array_equal(t0, t1) and np.array_equal(c0, c1)
