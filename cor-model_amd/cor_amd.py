@@ -18,6 +18,7 @@ DP = C.POINTER(C.c_double)
 
 COR_OK, COR_E_INVALID, COR_E_NO_DEVICE, COR_E_HIP, COR_E_UNSUPPORTED, COR_E_NOMEM = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_MIXED, PREC_MIXED_WAVE, PREC_MIXED_GROUP, PREC_MIXED_GROUP16, PREC_MIXED_GROUP32 = 0, 1, 2, 3, 4, 5
+PREC_FP64_WAVE, PREC_FP64_GROUP = 6, 7
 INTERNAL_NONE, INTERNAL_IGRF = 0, 1
 EXTERNAL_NONE, EXTERNAL_T96, EXTERNAL_TS05 = 0, 1, 2
 CRUSTAL_NONE, CRUSTAL_SH = 0, 1

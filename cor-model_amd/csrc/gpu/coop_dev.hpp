@@ -34,21 +34,23 @@ namespace cor {
 
 // Per-epoch lane tables from the T96 coefficient tables (lib/t96/t96.c function-level
 // citations at each family) and the epoch's amplitudes.  One thread; runs once per plan.
-__device__ inline void co_circ(CoopTab &T, int &l, const double R[9], double cx, double cy, double cz,
+template <class F>
+__device__ inline void co_circ(CoopTabT<F> &T, int &l, const double R[9], double cx, double cy, double cz,
                                double rl, double coef, int group) {
-    for (int k = 0; k < 9; k++) T.circ[l][k] = (float)R[k];
-    T.circ[l][9] = (float)cx;
-    T.circ[l][10] = (float)cy;
-    T.circ[l][11] = (float)cz;
-    T.circ[l][12] = (float)rl;
-    T.circ[l][13] = (float)coef;
-    T.circ[l][14] = (float)group;
-    T.circ[l][15] = 0.f;
+    for (int k = 0; k < 9; k++) T.circ[l][k] = (F)R[k];
+    T.circ[l][9] = (F)cx;
+    T.circ[l][10] = (F)cy;
+    T.circ[l][11] = (F)cz;
+    T.circ[l][12] = (F)rl;
+    T.circ[l][13] = (F)coef;
+    T.circ[l][14] = (F)group;
+    T.circ[l][15] = F(0);
     l++;
 }
 
 // crosslp (lib/t96/t96.c:883-898): two loops tilted by +-alpha about the x axis
-__device__ inline void co_crosslp(CoopTab &T, int &l, double xc, double rl, double cal, double sal,
+template <class F>
+__device__ inline void co_crosslp(CoopTabT<F> &T, int &l, double xc, double rl, double cal, double sal,
                                   double coef, int group) {
     const double R1[9] = {1, 0, 0, 0, cal, -sal, 0, sal, cal};
     const double R2[9] = {1, 0, 0, 0, cal, sal, 0, -sal, cal};
@@ -58,7 +60,8 @@ __device__ inline void co_crosslp(CoopTab &T, int &l, double xc, double rl, doub
 
 // loops4 (lib/t96/t96.c:1414-1470): four mirrored loops, each rotated about z by the
 // azimuth and then tilted about y; local = Tilt * Rz * (p - centre)
-__device__ inline void co_loops4(CoopTab &T, int &l, double xc, double yc, double zc, double rl, double st,
+template <class F>
+__device__ inline void co_loops4(CoopTabT<F> &T, int &l, double xc, double yc, double zc, double rl, double st,
                                  double ct, double sp, double cp, double coef, int group) {
     const double M[4][9] = {{cp, sp, 0, -sp, cp, 0, 0, 0, 1},
                             {cp, -sp, 0, sp, cp, 0, 0, 0, 1},
@@ -75,10 +78,11 @@ __device__ inline void co_loops4(CoopTab &T, int &l, double xc, double yc, doubl
     }
 }
 
-__device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
+template <class F>
+__device__ inline void build_coop_tab(const T96Params<double> &P, CoopTabT<F> &T) {
     using namespace t96;
-    float *z = &T.harm[0][0];
-    for (size_t k = 0; k < sizeof(CoopTab) / sizeof(float); k++) z[k] = 0.f;
+    F *z = &T.harm[0][0];
+    for (size_t k = 0; k < sizeof(CoopTabT<F>) / sizeof(F); k++) z[k] = F(0.);
     const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 
     // Cartesian harmonics: tailrc_96 shield sets (lib/t96/t96.c:501-617), birk1shld
@@ -90,19 +94,19 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
         for (int i = 0; i < 3; i++)
             for (int k = 0; k < 3; k++, l++) {
                 const double p = tab[9 + i], r = tab[12 + k];
-                T.harm[l][0] = (float)(1.0 / p);
-                T.harm[l][1] = (float)(1.0 / r);
-                T.harm[l][2] = (float)sqrt(1.0 / (p * p) + 1.0 / (r * r));
-                T.harm[l][3] = (float)(tab[3 * i + k] * set_amp[set]);
+                T.harm[l][0] = (F)(1.0 / p);
+                T.harm[l][1] = (F)(1.0 / r);
+                T.harm[l][2] = (F)sqrt(1.0 / (p * p) + 1.0 / (r * r));
+                T.harm[l][3] = (F)(tab[3 * i + k] * set_amp[set]);
             }
     }
     for (int i = 0; i < 4; i++)
         for (int k = 0; k < 4; k++, l++) {
             const double rp = 1.0 / k_b1s_py_d[i], rk = 1.0 / k_b1s_pz_d[k];
-            T.harm[l][0] = (float)rp;
-            T.harm[l][1] = (float)rk;
-            T.harm[l][2] = (float)sqrt(rp * rp + rk * rk);
-            T.harm[l][3] = (float)(k_b1s_amp_d[4 * i + k] * P.b1ampl);
+            T.harm[l][0] = (F)rp;
+            T.harm[l][1] = (F)rk;
+            T.harm[l][2] = (F)sqrt(rp * rp + rk * rk);
+            T.harm[l][3] = (F)(k_b1s_amp_d[4 * i + k] * P.b1ampl);
         }
     {
         const double ai[2] = {13.85650567, 14.90554500}, ak[2] = {10.21914434, 10.09021632};
@@ -110,10 +114,10 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
                                111.9448247 + -129.1957743, -110.7586562 + 126.5649012};
         for (int i = 0; i < 2; i++)
             for (int k = 0; k < 2; k++, l++) {
-                T.harm[l][0] = (float)(1.0 / ai[i]);
-                T.harm[l][1] = (float)(1.0 / ak[k]);
-                T.harm[l][2] = (float)sqrt(1.0 / (ai[i] * ai[i]) + 1.0 / (ak[k] * ak[k]));
-                T.harm[l][3] = (float)(amp[2 * i + k] * P.b2ampl);
+                T.harm[l][0] = (F)(1.0 / ai[i]);
+                T.harm[l][1] = (F)(1.0 / ak[k]);
+                T.harm[l][2] = (F)sqrt(1.0 / (ai[i] * ai[i]) + 1.0 / (ak[k] * ak[k]));
+                T.harm[l][3] = (F)(amp[2 * i + k] * P.b2ampl);
             }
     }
     // interconnection field (lib/t96/t96.c:236-271).  Per y harmonic i, the k = 0 term
@@ -131,7 +135,7 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
         const double rp[3] = {0.12503844191332988, 0.071597395204476152, 0.011080976889832743};
         const double rr[3] = {0.059675527522452816, 0.00098459526668325296, 0.00064371058922777988};
         for (int i = 0; i < 3; i++) {
-            float *t = T.icon[i];
+            F *t = T.icon[i];
             const double c0 = a[3 * i] * P.rimfampl, c1 = a[3 * i + 1] * P.rimfampl, c2 = a[3 * i + 2] * P.rimfampl;
             const double p2 = rp[i] * rp[i];
             const double s0 = sqrt(p2 + rr[0] * rr[0]), s1 = sqrt(p2 + rr[1] * rr[1]), s2 = sqrt(p2 + rr[2] * rr[2]);
@@ -144,7 +148,7 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
             const double v[21] = {rp[i], rr[0], s0, c0, s1, (rr[2] * rr[2] - rr[1] * rr[1]) / (s1 + s2), rr[2],
                                   c2 * s2, c2, c2 * rr[2], A[0], A[1] / 6., A[2] / 120., A[3] / 5040.,
                                   B[0], B[1] / 6., B[2] / 120., B[3] / 5040., B[1] / 2., B[2] / 24., B[3] / 720.};
-            for (int k = 0; k < 21; k++) t[k] = (float)v[k];
+            for (int k = 0; k < 21; k++) t[k] = (F)v[k];
         }
     }
     // conical sheets: ringcurr_96 (lib/t96/t96.c:322-381), taildisk_t96 (:383-443)
@@ -153,14 +157,14 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
         const double fd[4] = {-745796.7338, 1176470.141, -444610.529, -57508.01028};
         const double bd[4] = {7.9250000, 8.0850000, 8.4712500, 27.89500};
         for (int i = 0; i < 2; i++) {
-            T.cone[i][0] = (float)(fr[i] * P.rcampl);
-            T.cone[i][1] = (float)br[i];
+            T.cone[i][0] = (F)(fr[i] * P.rcampl);
+            T.cone[i][1] = (F)br[i];
         }
         for (int i = 0; i < 4; i++) {
-            T.cone[2 + i][0] = (float)(fd[i] * P.tampl2);
-            T.cone[2 + i][1] = (float)bd[i];
-            T.cone[2 + i][2] = 4.5f;
-            T.cone[2 + i][3] = 10.f;
+            T.cone[2 + i][0] = (F)(fd[i] * P.tampl2);
+            T.cone[2 + i][1] = (F)bd[i];
+            T.cone[2 + i][2] = F(4.5);
+            T.cone[2 + i][3] = F(10.);
         }
     }
     // cylindrical harmonics (lib/t96/t96.c:113-163)
@@ -168,29 +172,29 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
         const double a[12] = {0.24777, -27.003, -0.46815, 7.0637, -1.5918, -0.90317E-01,
                               57.522, 13.757, 2.0100, 10.458, 4.5798, 2.1695};
         for (int i = 0; i < 6; i++) {
-            T.cyl[i][0] = (float)(a[i] * P.xappa3);
-            T.cyl[i][1] = (float)a[i + 6];
-            T.cyl[i][2] = i < 3 ? 0.f : 1.f;
+            T.cyl[i][0] = (F)(a[i] * P.xappa3);
+            T.cyl[i][1] = (F)a[i + 6];
+            T.cyl[i][2] = i < 3 ? F(0.) : F(1.);
         }
     }
     // tail87 (lib/t96/t96.c:445-499): the sheet at z and its images at z -+ 40
-    T.t87[0][0] = 0.f;
-    T.t87[0][1] = (float)P.tampl3;
-    T.t87[1][0] = -40.f;
-    T.t87[1][1] = (float)(-0.5 * P.tampl3);
-    T.t87[2][0] = 40.f;
-    T.t87[2][1] = (float)(-0.5 * P.tampl3);
+    T.t87[0][0] = F(0.);
+    T.t87[0][1] = (F)P.tampl3;
+    T.t87[1][0] = -F(40.);
+    T.t87[1][1] = (F)(-0.5 * P.tampl3);
+    T.t87[2][0] = F(40.);
+    T.t87[2][1] = (F)(-0.5 * P.tampl3);
     // dipoles: diploop1 (lib/t96/t96.c:809-852) then condip1 (:920-1028), each a
     // point dipole of moment M at position D: field 30574 (3 (M.r) r - r^2 M) / r^5
     {
         int s = 0;
         auto dip = [&](double dx, double dy, double dz, double mx, double my, double mz) {
-            T.dip[s][0] = (float)dx;
-            T.dip[s][1] = (float)dy;
-            T.dip[s][2] = (float)dz;
-            T.dip[s][4] = (float)mx;
-            T.dip[s][5] = (float)my;
-            T.dip[s][6] = (float)mz;
+            T.dip[s][0] = (F)dx;
+            T.dip[s][1] = (F)dy;
+            T.dip[s][2] = (F)dz;
+            T.dip[s][4] = (F)mx;
+            T.dip[s][5] = (F)my;
+            T.dip[s][6] = (F)mz;
             s++;
         };
         for (int i = 0; i < 12; i++) {
@@ -236,14 +240,14 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
     {
         const double cc[5] = {154.185, -2.12446, 0.601735E-01, -.153954E-02, .355077E-04};
         for (int m = 1; m <= 5; m++) {
-            T.conic[m - 1][0] = (float)m;
-            T.conic[m - 1][1] = (float)(k_cd_c2_d[m - 1] * P.b1ampl);
-            T.conic[m - 1][2] = 0.16f;
-            T.conic[m - 1][3] = 0.f;
-            T.conic[4 + m][0] = (float)m;
-            T.conic[4 + m][1] = (float)(cc[m - 1] * P.b2ampl);
-            T.conic[4 + m][2] = 0.f;
-            T.conic[4 + m][3] = 1.f;
+            T.conic[m - 1][0] = (F)m;
+            T.conic[m - 1][1] = (F)(k_cd_c2_d[m - 1] * P.b1ampl);
+            T.conic[m - 1][2] = F(0.16);
+            T.conic[m - 1][3] = F(0.);
+            T.conic[4 + m][0] = (F)m;
+            T.conic[4 + m][1] = (F)(cc[m - 1] * P.b2ampl);
+            T.conic[4 + m][2] = F(0.);
+            T.conic[4 + m][3] = F(1.);
         }
     }
     // Region-2 sheet (lib/t96/t96.c:1472-1594): per component and latitude term k the
@@ -264,13 +268,13 @@ __device__ inline void build_coop_tab(const T96Params<double> &P, CoopTab &T) {
                                  {0.355625e-01, 0.318750e-01, 0.538750e-01, 8.42459695336914e-06}};
         for (int comp = 0; comp < 3; comp++)
             for (int k = 0; k < 5; k++) {
-                float *s = T.sheet[comp * 5 + k];
-                s[0] = (float)alpha[comp][k];
-                s[1] = (float)beta[comp][k];
-                s[2] = (float)gamma[comp][k];
-                s[3] = (float)comp;
-                for (int j = 0; j < 4; j++) s[4 + j] = (float)tf[comp][j];
-                for (int j = 0; j < 16; j++) s[8 + j] = (float)(k_r2s_d[comp * 80 + 16 * k + j] * P.b2ampl);
+                F *s = T.sheet[comp * 5 + k];
+                s[0] = (F)alpha[comp][k];
+                s[1] = (F)beta[comp][k];
+                s[2] = (F)gamma[comp][k];
+                s[3] = (F)comp;
+                for (int j = 0; j < 4; j++) s[4 + j] = (F)tf[comp][j];
+                for (int j = 0; j < 16; j++) s[8 + j] = (F)(k_r2s_d[comp * 80 + 16 * k + j] * P.b2ampl);
             }
     }
 }
@@ -284,8 +288,25 @@ template <int CTRL, int ROWS>
 __device__ __forceinline__ float dpp_add(float v) {
     return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
 }
-template <int N>
-__device__ __forceinline__ void wave_sum(float *p) {
+// the same for a double: both halves move with the same lane pattern
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, false);
+    return v + __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float readlane_any(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane_any(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int N, class F>
+__device__ __forceinline__ void wave_sum(F *p) {
 #pragma unroll
     for (int k = 0; k < N; k++) p[k] = dpp_add<0xB1, 0xF>(p[k]);    // quad_perm [1,0,3,2]
 #pragma unroll
@@ -299,7 +320,7 @@ __device__ __forceinline__ void wave_sum(float *p) {
 #pragma unroll
     for (int k = 0; k < N; k++) p[k] = dpp_add<0x143, 0xC>(p[k]);   // row_bcast:31 -> rows 2, 3
 #pragma unroll
-    for (int k = 0; k < N; k++) p[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[k]), 63));
+    for (int k = 0; k < N; k++) p[k] = readlane_any(p[k], 63);
 }
 
 // Sum of each of N values over a group of G lanes (G = 8: the three DPP steps within a
@@ -307,8 +328,8 @@ __device__ __forceinline__ void wave_sum(float *p) {
 // read by a lane permute; every lane of the group ends with the same bits, since each
 // step adds two values that the partner lane adds in the other order).  G = 64:
 // wave_sum.
-template <int N, int G>
-__device__ __forceinline__ void group_sum(float *p) {
+template <int N, int G, class F>
+__device__ __forceinline__ void group_sum(F *p) {
     static_assert(G == 8 || G == 16 || G == 32 || G == 64, "groups of 8, 16, 32 or 64 lanes");
     if constexpr (G == 64) {
         wave_sum<N>(p);
@@ -384,12 +405,29 @@ __device__ __forceinline__ void static_for(Fn &&f) {
     static_for_impl(std::make_integer_sequence<int, N>{}, f);
 }
 
-struct F3 {
-    float x, y, z;
-    __device__ __forceinline__ F3 operator+(const F3 &o) const { return {x + o.x, y + o.y, z + o.z}; }
+template <class F>
+struct Acc3 {
+    F x, y, z;
+    __device__ __forceinline__ Acc3 operator+(const Acc3 &o) const { return {x + o.x, y + o.y, z + o.z}; }
 };
 
-__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+// four consecutive table entries (a 16-byte-aligned row start)
+template <class F>
+struct Row4 {
+    F x, y, z, w;
+};
+__device__ __forceinline__ Row4<float> ld4(const float *p) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    return {v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ Row4<double> ld4(const double *p) {
+    const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+    return {a.x, a.y, b.x, b.y};
+}
+// 1/sqrt: the hardware estimate in FP32 (the conical harmonics' mixed-precision form),
+// correctly rounded root and division in FP64
+__device__ __forceinline__ float co_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double co_rsq(double x) { return 1.0 / sqrt(x); }
 
 // One term of the harmonic form shared by the shield sets, birk1shld, birk2shl and the
 // interconnection field: c e^{x sq} (-sq cos(y rp) sin(z rk), rp sin(y rp) sin(z rk),
@@ -407,44 +445,46 @@ __device__ __forceinline__ v3<T> co_harm(T x, T y, T z, T rp, T rk, T sq, T c) {
 
 // This lane's share of the sum FX of lib/t96/t96.c:91-93 plus the uniform terms that
 // finish the model (lib/t96/t96.c:4-111).
+template <class F>
 struct T96Share {
-    v3<float> fx;       // this lane's terms of FX
-    v3<float> q;        // dipole field (for the magnetopause blend)
-    float sigma, oimfy, oimfz;
+    v3<F> fx;           // this lane's terms of FX
+    v3<F> q;            // dipole field (for the magnetopause blend)
+    F sigma, oimfy, oimfz;
 };
 
 // The point and region state a T96 slot's terms need (uniform per trajectory).
+template <class F>
 struct T96Geo {
-    float xx, yy, zz;               // the point scaled by xappa
-    float X_i, Y_i, Z_i;            // the IMF-rotated point (interconnection field)
-    float lx, ly, lz, cxp, cyp, czp;   // Region-1 loop-basis and conical-basis points
-    float wl, wc, wo, ws, wi;       // blend weights: Region-1 loop, cone; Region-2 outer, sheet, inner
-    float xks;                      // Region-2 xi
+    F xx, yy, zz;                   // the point scaled by xappa
+    F X_i, Y_i, Z_i;                // the IMF-rotated point (interconnection field)
+    F lx, ly, lz, cxp, cyp, czp;    // Region-1 loop-basis and conical-basis points
+    F wl, wc, wo, ws, wi;           // blend weights: Region-1 loop, cone; Region-2 outer, sheet, inner
+    F xks;                          // Region-2 xi
     bool need_loop, need_cone;
 };
 
 // One Region-1 dipole (lib/t96/t96.c:809-852, 920-1028): 30574 w (3 (M.r) r - r^2 M) / r^5.
-__device__ __forceinline__ F3 coop_dipole(const CoopTab &T, int s, float w, float qx0, float qy0, float qz0) {
-    typedef float F;
-    const float4 dp = ld4(&T.dip[s][0]);
-    const float4 mm = ld4(&T.dip[s][4]);
+template <class F>
+__device__ __forceinline__ Acc3<F> coop_dipole(const CoopTabT<F> &T, int s, F w, F qx0, F qy0, F qz0) {
+    const Row4<F> dp = ld4(&T.dip[s][0]);
+    const Row4<F> mm = ld4(&T.dip[s][4]);
     const F X = qx0 - dp.x, Y = qy0 - dp.y, Z = qz0 - dp.z;
     const F r2 = X * X + Y * Y + Z * Z;
     const F qd = w * F(30574.0) / (r2 * r2 * Dm<F>::sqrt(r2));
     const F md = F(3.0) * (mm.x * X + mm.y * Y + mm.z * Z);
-    return F3{qd * (md * X - r2 * mm.x), qd * (md * Y - r2 * mm.y), qd * (md * Z - r2 * mm.z)};
+    return Acc3<F>{qd * (md * X - r2 * mm.x), qd * (md * Y - r2 * mm.y), qd * (md * Z - r2 * mm.z)};
 }
 
 // The terms of slot t (all families, in family order).  `base` is the smallest t of the
 // pass (a compile-time constant), so families whose terms all lie below it drop out.
-__device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const CoopTab &T, const T96Geo gm, const int base,
+template <class F>
+__device__ __forceinline__ Acc3<F> coop_t96_slot(const T96Params<F> &P, const CoopTabT<F> &T, const T96Geo<F> gm, const int base,
                                             const int t) {
     using namespace t96;
-    typedef float F;
     F px = 0, py = 0, pz = 0;
     // family 1: Cartesian harmonics (47)
     if (base < CO_NHARM && t < CO_NHARM) {
-        const float4 h = ld4(T.harm[t]);
+        const Row4<F> h = ld4(T.harm[t]);
         const v3<F> b = co_harm<F>(gm.xx, gm.yy, gm.zz, h.x, h.y, h.z, h.w);
         px += b.x;
         py += b.y;
@@ -453,7 +493,7 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
     // family 2: interconnection harmonics (3, one y harmonic each) at the
     // IMF-rotated point, rotated back (lib/t96/t96.c:66-72)
     if (base < CO_NICON && t < CO_NICON) {
-        const float *c = T.icon[t];
+        const F *c = T.icon[t];
         F sy, cy, sz, cz, s2, c2;
         Dm<F>::sincos(gm.Y_i * c[0], &sy, &cy);
         Dm<F>::sincos(gm.Z_i * c[1], &sz, &cz);
@@ -475,7 +515,7 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
     }
     // family 3: conical sheets of the ring current and the tail disk (6)
     if (base < CO_NCONE && t < CO_NCONE) {
-        const float4 c = ld4(T.cone[t]);
+        const Row4<F> c = ld4(T.cone[t]);
         const F f = c.x, beta = c.y, xs = gm.xx - c.z, d1 = c.w;
         const F y20 = gm.yy / F(20.0);
         const F dd = F(2.0) + d1 * (y20 * y20);
@@ -499,7 +539,7 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
     }
     // family 4: cylindrical harmonics (6)
     if (base < CO_NCYL && t < CO_NCYL) {
-        const float4 c = ld4(T.cyl[t]);
+        const Row4<F> c = ld4(T.cyl[t]);
         const F a = c.x, scale = c.y;
         F rho = Dm<F>::sqrt(gm.yy * gm.yy + gm.zz * gm.zz);
         F sinfi, cosfi;
@@ -531,7 +571,7 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
     }
     // family 5: tail87 sheet and images (3)
     if (base < CO_NT87 && t < CO_NT87) {
-        const float4 c = ld4(T.t87[t]);
+        const Row4<F> c = ld4(T.t87[t]);
         const F zl = gm.zz + c.x, w = c.y;
         const F xnx = F(-10.0) - gm.xx, xnx2 = xnx * xnx;
         const F xc1 = gm.xx + F(1.261), xc2 = gm.xx + F(0.663);
@@ -553,20 +593,20 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
     // family 6: Region-1 dipoles of the loop basis (21) at the loop point, then of the
     // conical basis (46) at the cone point
     if (base < CO_NDIP_LOOP && gm.need_loop && t < CO_NDIP_LOOP) {
-        const F3 d = coop_dipole(T, t, gm.wl, gm.lx, gm.ly, gm.lz);
+        const Acc3<F> d = coop_dipole(T, t, gm.wl, gm.lx, gm.ly, gm.lz);
         px += d.x;
         py += d.y;
         pz += d.z;
     }
     if (base < CO_NDIP - CO_NDIP_LOOP && gm.need_cone && t < CO_NDIP - CO_NDIP_LOOP) {
-        const F3 d = coop_dipole(T, CO_NDIP_LOOP + t, gm.wc, gm.cxp, gm.cyp, gm.czp);
+        const Acc3<F> d = coop_dipole(T, CO_NDIP_LOOP + t, gm.wc, gm.cxp, gm.cyp, gm.czp);
         px += d.x;
         py += d.y;
         pz += d.z;
     }
     // family 7: current loops (18; groups weighted by wl, wo, wi)
     if (base < CO_NCIRC && t < CO_NCIRC) {
-        const float *c = T.circ[t];
+        const F *c = T.circ[t];
         const int g = (int)c[14];
         const F w = g == 0 ? gm.wl : (g == 1 ? gm.wo : gm.wi);
         if (w != F(0.0)) {
@@ -583,7 +623,7 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
     }
     // family 8: conical harmonics (10: Region-1 conical basis, Region-2 inner)
     if (base < CO_NCONIC && t < CO_NCONIC) {
-        const float4 c = ld4(T.conic[t]);
+        const Row4<F> c = ld4(T.conic[t]);
         const bool g0 = c.w == F(0.0);
         const F w = g0 ? gm.wc : gm.wi;
         if (w != F(0.0)) {
@@ -592,14 +632,14 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
             // reciprocal square roots in place of the roots and divisions:
             // 1/(r s) = 1/ro, 1/ch^2 = 1/(0.5 (1 + cos)), 1/sh^2 = 1/(0.5 (1 - cos))
             const F ro2 = X * X + Y * Y;
-            const F irs = __builtin_amdgcn_rsqf(ro2);
+            const F irs = co_rsq(ro2);
             const F ro = ro2 * irs;
             const F cf = X * irs, sf = Y * irs;
             const F rr2 = ro2 + Z * Z;
-            const F ir = __builtin_amdgcn_rsqf(rr2);
+            const F ir = co_rsq(rr2);
             const F cc = Z * ir, sn = ro * ir;
             const F ch2 = F(0.5) * (F(1.0) + cc), sh2 = F(0.5) * (F(1.0) - cc);
-            const F ich = __builtin_amdgcn_rsqf(ch2), ish = __builtin_amdgcn_rsqf(sh2);
+            const F ich = co_rsq(ch2), ish = co_rsq(sh2);
             const F tnh = sh2 * ish * ich, cnh = ch2 * ich * ish;
             // cos/sin(m phi), tnh^m, cnh^m and the (m-1)-th powers
             F cfm = cf, sfm = sf, tm = tnh, cm = cnh, tm1 = F(1.0), cm1 = F(1.0);
@@ -626,7 +666,7 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
     }
     // family 9: Region-2 sheet (15)
     if (base < CO_NSHEET && gm.ws != F(0.0) && t < CO_NSHEET) {
-        const float *c = T.sheet[t];
+        const F *c = T.sheet[t];
         const int comp = (int)c[3];
         F t1, t2, t3;
         tfac(gm.xks, c[4], c[5], c[6], c[7], &t1, &t2, &t3);
@@ -660,15 +700,14 @@ __device__ __forceinline__ F3 coop_t96_slot(const T96Params<float> &P, const Coo
         py += k * (F(29.9996) * d6.y + F(262.886) * d7.y);
         pz += k * (F(29.9996) * d6.z + F(262.886) * d7.z);
     }
-    return F3{px, py, pz};
+    return Acc3<F>{px, py, pz};
 }
 
-template <int G>
-__device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, const CoopTab &T, float x, float y,
-                                                   float z, int gl) {
+template <class F, int G>
+__device__ __forceinline__ T96Share<F> coop_t96_share(const T96Params<F> &P, const CoopTabT<F> &T, F x, F y, F z,
+                                                      int gl) {
     using namespace t96;
-    typedef float F;
-    T96Share S;
+    T96Share<F> S;
     // ---- uniform preamble (lib/t96/t96.c:22-60)
     const F ys = y * P.ct - z * P.st, zs = z * P.ct + y * P.st;
     const F ysd = ys / F(10.0);
@@ -771,26 +810,26 @@ __device__ __forceinline__ T96Share coop_t96_share(const T96Params<float> &P, co
         }
     }
 
-    const T96Geo geo = {xx, yy, zz, xx, ys * P.xappa, zs * P.xappa, lx, ly, lz, cxp, cyp, czp,
+    const T96Geo<F> geo = {xx, yy, zz, xx, ys * P.xappa, zs * P.xappa, lx, ly, lz, cxp, cyp, czp,
                         wl, wc, wo, ws, wi, xks, need_loop, need_cone};
     // ---- this lane's block of slots, pairwise in slot order
     constexpr int NP = 64 / G;
     constexpr int LP = ilog2c(NP);
     const int r = lane_residue<G>(gl);
-    SlotTree<F3, NP> tree;
+    SlotTree<Acc3<F>, NP> tree;
     static_for<NP>([&](auto jc) __attribute__((always_inline)) {
         constexpr int j = decltype(jc)::value;
         constexpr int base = G * brevc(j, LP);
         tree.template push<j>(coop_t96_slot(P, T, geo, base, r + base));
     });
-    const F3 tot = tree.total();
+    const Acc3<F> tot = tree.total();
     S.fx = mk3<F>(tot.x, tot.y, tot.z);
     return S;
 }
 
 // Uniform end of lib/t96/t96.c:94-111 once FX is summed.
-__device__ __forceinline__ v3<float> t96_finish(const T96Share &S, v3<float> fx) {
-    typedef float F;
+template <class F>
+__device__ __forceinline__ v3<F> t96_finish(const T96Share<F> &S, v3<F> fx) {
     const v3<F> q = S.q;
     if (S.sigma >= F(1.08 + 0.005)) return mk3<F>(-q.x, S.oimfy - q.y, S.oimfz - q.z);
     if (S.sigma < F(1.08 - 0.005)) return fx;
@@ -806,18 +845,18 @@ __device__ __forceinline__ v3<float> t96_finish(const T96Share &S, v3<float> fx)
 // take a second order, m + 8, in a second pass): each lane advances the sectoral terms
 // to its order, then runs the degree recursion of its column.  Returns this lane's share
 // of (Br, Btheta, Bphi*sin).
+template <class F>
 struct IgrfShare {
-    float br, bt, bf;
-    float s, c, cf, sf;
+    F br, bt, bf;
+    F s, c, cf, sf;
     bool polar;
 };
 
-template <int GL>
-__device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G, const float *__restrict__ H,
-                                                     const float *__restrict__ REC, float x, float y, float z,
+template <class F, int GL>
+__device__ __forceinline__ IgrfShare<F> coop_igrf_share(const F *__restrict__ G, const F *__restrict__ H,
+                                                        const F *__restrict__ REC, F x, F y, F z,
                                                      int gl) {
-    typedef float F;
-    IgrfShare S;
+    IgrfShare<F> S;
     const F rho2 = x * x + y * y;
     const F r = Dm<F>::sqrt(rho2 + z * z);
     const F c = z / r;
@@ -894,24 +933,24 @@ __device__ __forceinline__ IgrfShare coop_igrf_share(const float *__restrict__ G
     }
     constexpr int NP = 64 / GL;
     constexpr int LP = ilog2c(NP);
-    SlotTree<F3, NP> tree;
+    SlotTree<Acc3<F>, NP> tree;
     static_for<NP>([&](auto jc) __attribute__((always_inline)) {
         constexpr int pass = brevc(decltype(jc)::value, LP);
         if constexpr (pass < NPI) {
-            tree.template push<decltype(jc)::value>(F3{cbr[pass], cbt[pass], cbf[pass]});
+            tree.template push<decltype(jc)::value>(Acc3<F>{cbr[pass], cbt[pass], cbf[pass]});
         } else {
-            tree.template push<decltype(jc)::value>(F3{F(0.), F(0.), F(0.)});
+            tree.template push<decltype(jc)::value>(Acc3<F>{F(0.), F(0.), F(0.)});
         }
     });
-    const F3 tot = tree.total();
+    const Acc3<F> tot = tree.total();
     S.br = tot.x;
     S.bt = tot.y;
     S.bf = tot.z;
     return S;
 }
 
-__device__ __forceinline__ v3<float> igrf_finish(const IgrfShare &S, float br, float bt, float bf) {
-    typedef float F;
+template <class F>
+__device__ __forceinline__ v3<F> igrf_finish(const IgrfShare<F> &S, F br, F bt, F bf) {
     F bp;
     if (S.polar) {
         bp = S.c < F(0.) ? -bf : bf;
@@ -927,24 +966,27 @@ __device__ __forceinline__ v3<float> igrf_finish(const IgrfShare &S, float br, f
 // GEO field [nT] of the model set at the GEO position geo_m / GSM position gsm_m [m],
 // identical in every lane of a group of GS lanes (gl: the lane's index in the group;
 // each group may be at its own position).
-template <int INT, int EXT, int GS>
-__device__ __forceinline__ v3<float> coop_field(const float *G, const float *H, const float *REC, const float *Af,
-                                                const T96Params<float> &P, const CoopTab &T, v3<double> geo_m,
-                                                v3<double> gsm_m, int gl) {
+// Position in Earth radii: a multiplication in FP32 (the same FP32 value to its last
+// bit), the reference's division in FP64.
+__device__ __forceinline__ float to_re(double x, float) { return (float)(x * K_INV_RE); }
+__device__ __forceinline__ double to_re(double x, double) { return x / K_RE; }
+
+template <class F, int INT, int EXT, int GS>
+__device__ __forceinline__ v3<F> coop_field(const F *G, const F *H, const F *REC, const F *Af,
+                                            const T96Params<F> &P, const CoopTabT<F> &T, v3<double> geo_m,
+                                            v3<double> gsm_m, int gl) {
     static_assert(GS == 8 || GS == 16 || GS == 32 || GS == 64, "groups of 8, 16, 32 or 64 lanes");
-    typedef float F;
-    IgrfShare I;
-    T96Share X;
+    IgrfShare<F> I;
+    T96Share<F> X;
     F p[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (INT == INT_IGRF) {
-        I = coop_igrf_share<GS>(G, H, REC, (F)(geo_m.x * K_INV_RE), (F)(geo_m.y * K_INV_RE), (F)(geo_m.z * K_INV_RE),
-                                gl);
+        I = coop_igrf_share<F, GS>(G, H, REC, to_re(geo_m.x, F(0)), to_re(geo_m.y, F(0)), to_re(geo_m.z, F(0)), gl);
         p[0] = I.br;
         p[1] = I.bt;
         p[2] = I.bf;
     }
     if constexpr (EXT == EXT_T96) {
-        X = coop_t96_share<GS>(P, T, (F)(gsm_m.x * K_INV_RE), (F)(gsm_m.y * K_INV_RE), (F)(gsm_m.z * K_INV_RE), gl);
+        X = coop_t96_share<F, GS>(P, T, to_re(gsm_m.x, F(0)), to_re(gsm_m.y, F(0)), to_re(gsm_m.z, F(0)), gl);
         p[3] = X.fx.x;
         p[4] = X.fx.y;
         p[5] = X.fx.z;
@@ -963,10 +1005,11 @@ __device__ __forceinline__ v3<float> coop_field(const float *G, const float *H, 
 }
 
 // Copy the lane tables into the workgroup's LDS (all threads of the block).
-__device__ __forceinline__ void stage_coop(const CoopTab *__restrict__ src, CoopTab *dst) {
+template <class F>
+__device__ __forceinline__ void stage_coop(const CoopTabT<F> *__restrict__ src, CoopTabT<F> *dst) {
     const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
     uint32_t *d = reinterpret_cast<uint32_t *>(dst);
-    for (int k = threadIdx.x; k < (int)(sizeof(CoopTab) / 4); k += blockDim.x) d[k] = s[k];
+    for (int k = threadIdx.x; k < (int)(sizeof(CoopTabT<F>) / 4); k += blockDim.x) d[k] = s[k];
     __syncthreads();
 }
 

@@ -414,7 +414,11 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     if (rc) return rc;
     int intk, ext, cru;
     if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
-    const void *kern = opts->precision == COR_PREC_FP64 ? trace_kernel_fp64(intk, ext, cru)
+    // FP64 with the cooperative tail (opt-in) has a kernel of its own; model sets the
+    // cooperative evaluator does not cover run per-lane
+    const bool fp64_coop = opts->precision == COR_PREC_FP64 && opts->coop > 0 && trace_kernel_fp64_coop(intk, ext, cru);
+    const void *kern = opts->precision == COR_PREC_FP64 ? (fp64_coop ? trace_kernel_fp64_coop(intk, ext, cru)
+                                                                     : trace_kernel_fp64(intk, ext, cru))
                                                        : trace_kernel_mixed(intk, ext, cru);
     if (!kern) return COR_E_UNSUPPORTED;
 
@@ -503,7 +507,7 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     }
 
     // the cooperative evaluator covers IGRF and T96 (TS05 and SH traces stay per-lane)
-    p->coop_on = p->prec == COR_PREC_MIXED && opts->coop >= 0 && ext != EXT_TS05 && cru == CRU_NONE;
+    p->coop_on = (p->prec == COR_PREC_MIXED ? opts->coop >= 0 : opts->coop > 0) && ext != EXT_TS05 && cru == CRU_NONE;
     p->handover_steps = opts->handover_steps > 0 ? (uint32_t)opts->handover_steps : COOP_HANDOVER_DEFAULT;
 
     // persistent grid: as many resident blocks as the occupancy allows, but no more
@@ -854,18 +858,20 @@ int cor_gpu_rk_step(const double *h, const double *d, const double *field, doubl
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
                   double *geo_nT, int64_t n, int32_t precision) {
     if (!epoch || n < 0 || (n > 0 && (!gsm_m || !geo_nT))) return COR_E_INVALID;
-    if (precision < COR_PREC_FP64 || precision > COR_PREC_MIXED_GROUP32) return COR_E_INVALID;
+    if (precision < COR_PREC_FP64 || precision > COR_PREC_FP64_GROUP) return COR_E_INVALID;
     const bool coop = precision >= COR_PREC_MIXED_WAVE;
-    const int lanes = precision == COR_PREC_MIXED_GROUP     ? COOP_GROUP
-                      : precision == COR_PREC_MIXED_GROUP16 ? 16
-                      : precision == COR_PREC_MIXED_GROUP32 ? 32
-                                                            : 64;
+    const bool coop64 = precision >= COR_PREC_FP64_WAVE;
+    const int lanes = precision == COR_PREC_MIXED_GROUP || precision == COR_PREC_FP64_GROUP ? COOP_GROUP
+                      : precision == COR_PREC_MIXED_GROUP16                                  ? 16
+                      : precision == COR_PREC_MIXED_GROUP32                                  ? 32
+                                                                                             : 64;
     int rc = ensure_init();
     if (rc) return rc;
     int intk, ext, cru;
     if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
     const void *kern = precision == COR_PREC_FP64    ? field_kernel_fp64(intk, ext, cru)
                        : precision == COR_PREC_MIXED ? field_kernel_mixed(intk, ext, cru)
+                       : coop64                      ? field_kernel_coop_fp64(intk, ext, cru, lanes)
                                                      : field_kernel_coop(intk, ext, cru, lanes);
     if (!kern) return COR_E_UNSUPPORTED;
     if (n == 0) return COR_OK;

@@ -50,17 +50,19 @@ constexpr int CO_NCIRC = 18;       // current loops: Region-1 loop basis 3, Regi
 constexpr int CO_NCONIC = 10;      // conical harmonics: Region-1 5, Region-2 inner 5
 constexpr int CO_NSHEET = 15;      // Region-2 sheet: 3 components x 5 latitude terms
 
-struct alignas(16) CoopTab {   // rows are read as float4
-    float harm[64][4];           // 1/p, 1/r, sqrt(1/p^2 + 1/r^2), coefficient x amplitude
-    float cone[8][4];            // f x amplitude, beta, x offset, dd slope (0 ring current, 10 disk)
-    float cyl[8][4];             // a x xappa^3, scale, kind (0: j1 terms, 1: mixed terms), 0
-    float t87[4][4];             // z offset, weight x amplitude, 0, 0
-    float dip[68][8];            // position (3), 0, moment x amplitude (3), 0
-    float circ[24][16];          // rotation rows (9), centre (3), radius, coefficient x amplitude, group, 0
-    float conic[16][4];          // order, coefficient x amplitude, x offset, group
-    float sheet[16][24];         // alpha, beta, gamma, component, d1, d2, d3, d3^4, 16 coefficients
-    float icon[4][24];           // per y harmonic: the k = 0 term and the series of the k = 1, 2 pair
+template <class F>
+struct alignas(16) CoopTabT {  // rows are read 16 bytes at a time
+    F harm[64][4];           // 1/p, 1/r, sqrt(1/p^2 + 1/r^2), coefficient x amplitude
+    F cone[8][4];            // f x amplitude, beta, x offset, dd slope (0 ring current, 10 disk)
+    F cyl[8][4];             // a x xappa^3, scale, kind (0: j1 terms, 1: mixed terms), 0
+    F t87[4][4];             // z offset, weight x amplitude, 0, 0
+    F dip[68][8];            // position (3), 0, moment x amplitude (3), 0
+    F circ[24][16];          // rotation rows (9), centre (3), radius, coefficient x amplitude, group, 0
+    F conic[16][4];          // order, coefficient x amplitude, x offset, group
+    F sheet[16][24];         // alpha, beta, gamma, component, d1, d2, d3, d3^4, 16 coefficients
+    F icon[4][24];           // per y harmonic: the k = 0 term and the series of the k = 1, 2 pair
 };
+typedef CoopTabT<float> CoopTab;
 
 struct EpochDev {
     double G[NCOEF], H[NCOEF], REC[NCOEF];
@@ -76,7 +78,8 @@ struct EpochDev {
     int32_t pad;
     const double *shG, *shH;     // SH tables [sh_nm][sh_nm] in their own device block
     const float *shGf, *shHf;
-    CoopTab coop;
+    CoopTab coop;                // mixed precision
+    CoopTabT<double> coopd;      // FP64 cooperative tail
 };
 
 // State of a trajectory handed from the per-lane mode to the cooperative mode, taken

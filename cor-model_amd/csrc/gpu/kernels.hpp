@@ -26,12 +26,15 @@ struct RkStepArgs {
 // is not compiled in.
 const void *trace_kernel_fp64(int intk, int ext, int cru);
 const void *trace_kernel_mixed(int intk, int ext, int cru);
+// the FP64 tracer with the cooperative tail (IGRF / T96 model sets; nullptr otherwise)
+const void *trace_kernel_fp64_coop(int intk, int ext, int cru);
 const void *field_kernel_fp64(int intk, int ext, int cru);
 const void *field_kernel_mixed(int intk, int ext, int cru);
 // Mixed-precision field batch evaluated like the tracer's cooperative mode (one
 // wavefront per point), and the one-thread kernel that builds the cooperative lane
 // tables of an epoch in place (EpochDev::coop).
 const void *field_kernel_coop(int intk, int ext, int cru, int lanes);
+const void *field_kernel_coop_fp64(int intk, int ext, int cru, int lanes);
 const void *coop_build_kernel();
 const void *rk_step_kernel_fp64();
 

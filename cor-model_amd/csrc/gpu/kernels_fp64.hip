@@ -5,8 +5,14 @@ namespace cor {
 const void *trace_kernel_fp64(int intk, int ext, int cru) {
     return trace_kernel_table<PREC_FP64>(intk, ext, cru);
 }
+const void *trace_kernel_fp64_coop(int intk, int ext, int cru) {
+    return trace_kernel_coop_table<PREC_FP64>(intk, ext, cru);
+}
 const void *field_kernel_fp64(int intk, int ext, int cru) {
     return field_kernel_table<PREC_FP64>(intk, ext, cru);
+}
+const void *field_kernel_coop_fp64(int intk, int ext, int cru, int lanes) {
+    return field_coop_table_lanes<PREC_FP64>(intk, ext, cru, lanes);
 }
 
 // One RK step per state with the tracer's FP64 step (src/trajectory.c:275-356).

@@ -29,15 +29,16 @@ __global__ void __launch_bounds__(256) field_batch_kernel(FieldBatchArgs a) {
 }
 
 // The same batch with the cooperative evaluator as the tracer's cooperative mode runs
-// it: one group of GS lanes (COOP_GROUP or 64) per point, points strided over the grid's groups.
-// Every lane of a wavefront takes part in each pass (the group sums need all lanes);
-// groups past the end evaluate the last point again and write nothing.
-template <int INT, int EXT, int CRU, int GS>
+// it: one group of GS lanes (8, 16, 32 or 64) per point, points strided over the grid's
+// groups.  Every lane of a wavefront takes part in each pass (the group sums need all
+// lanes); groups past the end evaluate the last point again and write nothing.
+template <int PREC, int INT, int EXT, int CRU, int GS>
 __global__ void __launch_bounds__(256) field_coop_kernel(FieldBatchArgs a) {
-    __shared__ float s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
-    __shared__ CoopTab s_coop;
-    const EpochView<float> E = stage_epoch<float>(a.epoch, s_g, s_h, s_rec);
-    stage_coop(&a.epoch->coop, &s_coop);
+    typedef typename FieldT<PREC>::T FT;
+    __shared__ FT s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
+    __shared__ CoopTabT<FT> s_coop;
+    const EpochView<FT> E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
+    stage_coop(coop_tab(a.epoch, FT(0)), &s_coop);
     const int gl = __lane_id() & (GS - 1);
     const int64_t groups = (int64_t)gridDim.x * (blockDim.x / GS);
     const int64_t first = (int64_t)blockIdx.x * (blockDim.x / GS) + threadIdx.x / GS;
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(256) field_coop_kernel(FieldBatchArgs a) {
         const int64_t ic = i < a.n ? i : a.n - 1;
         const v3<double> gsm = mk3(a.gsm_m[3 * ic], a.gsm_m[3 * ic + 1], a.gsm_m[3 * ic + 2]);
         const v3<double> geo = rot_gsm_to_geo(E.A, gsm);
-        const v3<float> f = coop_field<INT, EXT, GS>(E.G, E.H, E.REC, E.Af, E.t96, s_coop, geo, gsm, gl);
+        const v3<FT> f = coop_field<FT, INT, EXT, GS>(E.G, E.H, E.REC, rot_rows(E), E.t96, s_coop, geo, gsm, gl);
         if (gl == 0 && i < a.n) {
             a.geo_nT[3 * i] = f.x;
             a.geo_nT[3 * i + 1] = f.y;
@@ -71,6 +72,29 @@ __global__ void __launch_bounds__(256) field_coop_kernel(FieldBatchArgs a) {
     X(INT_IGRF, EXT_T96, CRU_SH)           \
     X(INT_IGRF, EXT_TS05, CRU_SH)
 
+template <int PREC, int GS>
+const void *field_coop_table(int intk, int ext, int cru);
+
+template <int PREC>
+const void *field_coop_table_lanes(int intk, int ext, int cru, int lanes) {
+    switch (lanes) {
+    case 64: return field_coop_table<PREC, 64>(intk, ext, cru);
+    case 32: return field_coop_table<PREC, 32>(intk, ext, cru);
+    case 16: return field_coop_table<PREC, 16>(intk, ext, cru);
+    default: return field_coop_table<PREC, COOP_GROUP>(intk, ext, cru);
+    }
+}
+
+// the FP64 tracer with the cooperative tail (the cooperative model sets only)
+template <int PREC>
+const void *trace_kernel_coop_table(int intk, int ext, int cru) {
+#define COR_PICK(I, E, C) \
+    if (intk == I && ext == E && cru == C) return (const void *)&trace_kernel<PREC, I, E, C, true>;
+    COR_COOP_MODEL_SETS(COR_PICK)
+#undef COR_PICK
+    return nullptr;
+}
+
 template <int PREC>
 const void *trace_kernel_table(int intk, int ext, int cru) {
 #define COR_PICK(I, E, C) \
@@ -89,10 +113,10 @@ const void *field_kernel_table(int intk, int ext, int cru) {
     return nullptr;
 }
 
-template <int GS>
+template <int PREC, int GS>
 const void *field_coop_table(int intk, int ext, int cru) {
 #define COR_PICK(I, E, C) \
-    if (intk == I && ext == E && cru == C) return (const void *)&field_coop_kernel<I, E, C, GS>;
+    if (intk == I && ext == E && cru == C) return (const void *)&field_coop_kernel<PREC, I, E, C, GS>;
     COR_COOP_MODEL_SETS(COR_PICK)
 #undef COR_PICK
     return nullptr;

@@ -51,6 +51,11 @@ __device__ __forceinline__ const TS05Params<float> *ts05_of(const EpochDev *E, f
 
 __device__ __forceinline__ const T96Params<double> &t96_of(const EpochDev *E, double) { return E->t96d; }
 __device__ __forceinline__ const T96Params<float> &t96_of(const EpochDev *E, float) { return E->t96f; }
+__device__ __forceinline__ const CoopTabT<double> *coop_tab(const EpochDev *E, double) { return &E->coopd; }
+__device__ __forceinline__ const CoopTabT<float> *coop_tab(const EpochDev *E, float) { return &E->coop; }
+// the GSM -> GEO rotation in the field type (the cooperative evaluator's last step)
+__device__ __forceinline__ const double *rot_rows(const EpochView<double> &E) { return E.A; }
+__device__ __forceinline__ const float *rot_rows(const EpochView<float> &E) { return E.Af; }
 __device__ __forceinline__ const double *igrf_tab(const EpochDev *E, int k, double) {
     return k == 0 ? E->G : (k == 1 ? E->H : E->REC);
 }
@@ -375,7 +380,8 @@ template <class EV>
 __device__ __forceinline__ int attempts_fp64(const TraceArgs &a, const EV &E, v3<double> fld,
                                              v3<double> &pos, v3<double> &vel, double &v, double b,
                                              double zn, int &nk0, int nk1, unsigned int &step,
-                                             uint32_t &attempts, double &time, double &alength, int64_t gid) {
+                                             uint32_t &attempts, double &time, double &alength, int64_t gid,
+                                             bool writer = true) {
     for (;;) {
         step++;
         const double vc = v / K_C;
@@ -415,7 +421,7 @@ __device__ __forceinline__ int attempts_fp64(const TraceArgs &a, const EV &E, v3
         const double paus = pax * pax + paz / 459.6736;
         if (r >= 25. || paus > 1.) {
             if ((r >= 25. && fabs(r - 25.) < 0.002) || (r <= 25. && paus > 1. && fabs(paus - 1.) < 0.002)) {
-                write_exit(a, gid, pos, vel, v, r, time, alength);
+                if (writer) write_exit(a, gid, pos, vel, v, r, time, alength);
                 return ST_ALLOWED;
             }
             nk0 = (int)((unsigned)nk0 * 2u);
@@ -597,9 +603,9 @@ __device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool ac
 // j evaluates the j-th live trajectory (leads: the first lanes of the bookkeeping groups
 // that hold one); each bookkeeping group then reads its trajectory's field from that
 // field group, so all its lanes hold the same bits.
-template <int INT, int EXT, int FG>
-__device__ __forceinline__ v3<float> coop_field_spread(const EpochView<float> &E, const CoopTab &T, uint64_t leads,
-                                                      v3<double> pos, int lane, int lead) {
+template <class FT, int INT, int EXT, int FG>
+__device__ __forceinline__ v3<FT> coop_field_spread(const EpochView<FT> &E, const CoopTabT<FT> &T, uint64_t leads,
+                                                    v3<double> pos, int lane, int lead) {
     const int fg = lane / FG;
     uint64_t m = leads;
 #pragma unroll
@@ -608,7 +614,7 @@ __device__ __forceinline__ v3<float> coop_field_spread(const EpochView<float> &E
     const int src = (m ? __ffsll((unsigned long long)m) : __ffsll((unsigned long long)leads)) - 1;
     const v3<double> p1 = mk3(__shfl(pos.x, src), __shfl(pos.y, src), __shfl(pos.z, src));
     const v3<double> gsm = rot_geo_to_gsm(E.A, p1);
-    const v3<float> fo = coop_field<INT, EXT, FG>(E.G, E.H, E.REC, E.Af, E.t96, T, p1, gsm, lane & (FG - 1));
+    const v3<FT> fo = coop_field<FT, INT, EXT, FG>(E.G, E.H, E.REC, rot_rows(E), E.t96, T, p1, gsm, lane & (FG - 1));
     const int r = __popcll(leads & ((1ull << lead) - 1ull));
     const int from = (r < 64 / FG ? r : 0) * FG;
     return mk3(__shfl(fo.x, from), __shfl(fo.y, from), __shfl(fo.z, from));
@@ -637,8 +643,10 @@ __device__ __forceinline__ v3<float> coop_field_spread(const EpochView<float> &E
 // trajectory can run to the step limit at the slowest step of any mode, so it cannot
 // fire while a trajectory is still running -- an idle wavefront retires and the host
 // reports the shortfall.  Workers never wait for one another and parkers never wait.
-template <int INT, int EXT>
-__device__ bool coop_worker(const TraceArgs &a, const EpochView<float> &E, const CoopTab &T, int lane) {
+template <int PREC, int INT, int EXT>
+__device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<PREC>::T> &E,
+                            const CoopTabT<typename FieldT<PREC>::T> &T, int lane) {
+    typedef typename FieldT<PREC>::T FT;
     constexpr int GS = COOP_GROUP;
     const int gl = lane & (GS - 1);
     const int lead = lane & ~(GS - 1);            // first lane of this lane's group
@@ -798,25 +806,31 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<float> &E, const
         // ---- 3. one step of each trajectory.  The field: a lone trajectory with all 64
         // lanes, two with 32 lanes each, three or four with 16, more with their own
         // group of COOP_GROUP lanes (groups without one evaluate at their last point).
-        v3<float> f;
+        v3<FT> f;
         const int k_live = __popcll(has_leads);
         if (k_live == 1) {
             const int src = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)has_leads) - 1);
             const v3<double> p1 = mk3(read_lane_d(pos.x, src), read_lane_d(pos.y, src), read_lane_d(pos.z, src));
             const v3<double> gsm = rot_geo_to_gsm(E.A, p1);
-            f = coop_field<INT, EXT, 64>(E.G, E.H, E.REC, E.Af, E.t96, T, p1, gsm, lane);
+            f = coop_field<FT, INT, EXT, 64>(E.G, E.H, E.REC, rot_rows(E), E.t96, T, p1, gsm, lane);
         } else if (k_live == 2) {
-            f = coop_field_spread<INT, EXT, 32>(E, T, has_leads, pos, lane, lead);
+            f = coop_field_spread<FT, INT, EXT, 32>(E, T, has_leads, pos, lane, lead);
         } else if (k_live <= 4) {
-            f = coop_field_spread<INT, EXT, 16>(E, T, has_leads, pos, lane, lead);
+            f = coop_field_spread<FT, INT, EXT, 16>(E, T, has_leads, pos, lane, lead);
         } else {
             const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
-            f = coop_field<INT, EXT, GS>(E.G, E.H, E.REC, E.Af, E.t96, T, pos, gsm, gl);
+            f = coop_field<FT, INT, EXT, GS>(E.G, E.H, E.REC, rot_rows(E), E.t96, T, pos, gsm, gl);
         }
         if (has) {
             evals++;
-            const int status = attempts_mixed(a, E, f, pos, vel, v, inv_b_mixed(f), zn, nk0, nk1, step, attempts, time, alength,
-                                              gid, is_lead);
+            int status;
+            if constexpr (PREC == PREC_FP64) {
+                const double b = 1.E-09 * sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
+                status = attempts_fp64(a, E, f, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid, is_lead);
+            } else {
+                status = attempts_mixed(a, E, f, pos, vel, v, inv_b_mixed(f), zn, nk0, nk1, step, attempts, time, alength,
+                                        gid, is_lead);
+            }
             if (status >= 0) {
                 if (is_lead) {
                     write_end(a, gid, status, step, attempts, handover);
@@ -837,19 +851,21 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<float> &E, const
 #define COR_WAVES_PER_EU 2
 #endif
 
-template <int PREC, int INT, int EXT, int CRU>
+// COOP: the kernel carries the cooperative worker (mixed precision always; FP64 in a
+// kernel of its own, so the FP64 per-lane kernel keeps its registers free of it).
+template <int PREC, int INT, int EXT, int CRU, bool COOP = PREC == PREC_MIXED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COR_WAVES_PER_EU)))
 trace_kernel(TraceArgs a) {
     typedef typename FieldT<PREC>::T FT;
     __shared__ FT s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
     const EpochView<FT> E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
     const int lane = __lane_id();
-    const CoopTab *ctab = nullptr;
-    constexpr bool kCoop = PREC == PREC_MIXED && EXT != EXT_TS05 && CRU == CRU_NONE;   // the cooperative evaluator's models
+    const CoopTabT<FT> *ctab = nullptr;
+    constexpr bool kCoop = COOP && EXT != EXT_TS05 && CRU == CRU_NONE;   // the cooperative evaluator's models
     if constexpr (kCoop) {
-        __shared__ CoopTab s_coop;
+        __shared__ CoopTabT<FT> s_coop;
         if (a.coop_on) {
-            stage_coop(&a.epoch->coop, &s_coop);
+            stage_coop(coop_tab(a.epoch, FT(0)), &s_coop);
             if (lane == 0) atomicAdd(&a.coop->running, 1);
         }
         ctab = &s_coop;
@@ -1033,7 +1049,7 @@ trace_kernel(TraceArgs a) {
                     if (converting) atomicSub(&a.coop->converting, 1);
                     __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                if (coop_worker<INT, EXT>(a, E, *ctab, lane)) break;
+                if (coop_worker<PREC, INT, EXT>(a, E, *ctab, lane)) break;
                 continue;
             }
         }

@@ -36,9 +36,12 @@ enum {
 /* COR_PREC_MIXED_WAVE / _GROUP / _GROUP16 / _GROUP32: cor_gpu_field only -- the
  * mixed-precision model evaluated as the tracer's cooperative mode evaluates it: one
  * wavefront (64 lanes) per point, one group of 8 lanes (eight trajectories to a
- * wavefront), or 16 or 32 lanes (four or two to a wavefront). */
+ * wavefront), or 16 or 32 lanes (four or two to a wavefront).  COR_PREC_FP64_WAVE /
+ * _GROUP: the same evaluator in FP64 (the FP64 tracer's cooperative tail), 64 or 8
+ * lanes per point. */
 enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1, COR_PREC_MIXED_WAVE = 2, COR_PREC_MIXED_GROUP = 3,
-       COR_PREC_MIXED_GROUP16 = 4, COR_PREC_MIXED_GROUP32 = 5 };
+       COR_PREC_MIXED_GROUP16 = 4, COR_PREC_MIXED_GROUP32 = 5, COR_PREC_FP64_WAVE = 6,
+       COR_PREC_FP64_GROUP = 7 };
 enum { COR_INTERNAL_NONE = 0, COR_INTERNAL_IGRF = 1 };
 enum { COR_EXTERNAL_NONE = 0, COR_EXTERNAL_T96 = 1, COR_EXTERNAL_TS05 = 2 };
 enum { COR_CRUSTAL_NONE = 0, COR_CRUSTAL_SH = 1 };
@@ -77,8 +80,10 @@ typedef struct {
     double particle_mass;     /* DEF_HM0_P or DEF_HM0_E */
     int32_t precision;        /* COR_PREC_FP64 or COR_PREC_MIXED */
     int32_t queue_order;      /* 0 = rigidity order, 1 = longest-expected first */
-    int32_t coop;             /* COR_PREC_MIXED with IGRF and/or T96: 0 = the cooperative mode
-                               * (the library default), negative = per-lane mode only */
+    int32_t coop;             /* the cooperative mode (IGRF and/or T96 model sets): 0 = the
+                               * library default (on for COR_PREC_MIXED; off for COR_PREC_FP64,
+                               * which then keeps the reference's operation order throughout),
+                               * positive = on, negative = per-lane mode only */
     int32_t handover_steps;   /* with the cooperative mode: accepted steps a trajectory runs in
                                * the per-lane mode (one lane each) before it continues in the
                                * cooperative mode (a group of lanes each); 0 = the library

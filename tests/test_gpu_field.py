@@ -124,6 +124,29 @@ def test_cooperative_field_same_bits_in_every_group_width(cor, name, kinds, parm
         assert same.all(), np.argwhere(~same.all(axis=1))[:5].ravel()
 
 
+@pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
+def test_cooperative_field_fp64(cor, oracle, name, kinds, parmod, pts):
+    """The cooperative evaluator in FP64 (the FP64 tracer's cooperative tail): the same
+    terms and canonical order as the mixed one, so 64 and 8 lanes per point give the
+    same bits; against the oracle the error is FP64 rounding in another summation order
+    (T96's shielding fits cancel by up to 1e5, hence the bar relative to |B| + the
+    dipole scale: 1e-10)."""
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=parmod)
+    _, po = oracle.recalc(2020, 1, 1, 1, 0, 0, 0)
+    if parmod:
+        for k, v in enumerate(parmod):
+            po.PARMOD[k] = v
+    ms = cor.model_set(**kinds)
+    w = cor.field(ms, p, pts, cor.PREC_FP64_WAVE)
+    g = cor.field(ms, p, pts, cor.PREC_FP64_GROUP)
+    assert np.array_equal(w, g)
+    want = oracle.field(L.models(**kinds), po, pts)
+    r = np.linalg.norm(pts, axis=1) / RE
+    err = np.linalg.norm(w - want, axis=1) / (np.linalg.norm(want, axis=1) + 30574.0 / r ** 3)
+    print(name, "fp64 cooperative max rel err", err.max())
+    assert err.max() < 1e-10
+
+
 TS05_CASES = [
     ("ts05", dict(internal=0, external=2), [2, -20, 0, -2, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], box_points(6, 4000)),
     ("igrf+ts05 storm", dict(internal=1, external=2), [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5],

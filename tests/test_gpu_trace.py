@@ -134,6 +134,35 @@ def test_t96_sites_cooperative(cor, oracle, lat, lon):
     compare(row, tg, cg, tc, cc, rc_tol=rc_tol, dir_tol=0.5, dir_margin=1.0, exit_tol=5.0)
 
 
+@pytest.mark.parametrize("lat,lon", [(0.0, 0.0), (50.0, 120.0), (-35.0, 290.0)])
+def test_t96_sites_fp64_cooperative(cor, oracle, lat, lon):
+    """The FP64 tracer with its cooperative tail (opt-in), every trajectory cooperative
+    from its first step: the FP64 bar of the per-lane FP64 test at the same sites (flags
+    identical away from the band, Rc within 0.05 GV of the oracle)."""
+    osc = scan(lat, lon, rig=0.5, dl=0.05, rk=20.0)
+    row, tg, cg, tc, cc = run_both(cor, oracle, osc, dict(internal=1, external=1), parmod=[2, -20, 0, -2],
+                                   coop=1, handover_steps=1)
+    print(lat, lon, "fp64 cooperative", cg, "cpu", cc)
+    # handed over after the first accepted step (recorded as 2), unless it ended there
+    assert np.all(np.isin(tg["handover"], (0, 2))) and (tg["handover"] == 2).mean() > 0.9
+    compare(row, tg, cg, tc, cc, rc_tol=0.05)
+
+
+def test_fp64_cooperative_is_deterministic(cor):
+    """The FP64 tracer with the cooperative tail: bit-identical records and cutoffs in
+    repeated runs and both queue orders (handover after 300 steps)."""
+    sites = cor.grid_sites()[::37]
+    rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
+    ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
+    out = [cor.trace(ms, p, rows, step_limit=100000, precision=cor.PREC_FP64, coop=1, handover_steps=300,
+                     queue_order=q) for q in (1, 1, 0)]
+    print(len(rows), "scans,", int((out[0][0]["handover"] > 0).sum()), "trajectories cooperative")
+    assert int((out[0][0]["handover"] > 0).sum()) > 0
+    for t, c in out[1:]:
+        assert np.array_equal(t, out[0][0]) and np.array_equal(c, out[0][1])
+
+
 def test_cooperative_handover_many_scans(cor):
     """Many scans through the handover: every trajectory finishes exactly once (status
     set, steps counted) whichever mode ran it, a handed-over trajectory left the
