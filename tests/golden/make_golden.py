@@ -11,6 +11,7 @@ reference sources (make -C oracle ref first):
     python tests/golden/make_golden.py field    (the field-map run only)
     python tests/golden/make_golden.py track    (the trajectory-mode runs only)
     python tests/golden/make_golden.py internal (every internal model's epoch setup only)
+    python tests/golden/make_golden.py grid5    (five grid sites at the benchmark's resolution only)
 """
 import sys
 from pathlib import Path
@@ -19,6 +20,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parent))
+sys.path.insert(0, str(HERE.parent.parent / "cor-model_amd"))
 import oracle_lib as L  # noqa: E402
 
 EPOCHS = [(2020, 1, 1, 1, 0, 0, 0), (2020, 3, 15, 75, 12, 30, 5), (1965, 6, 1, 152, 3, 0, 0),
@@ -161,6 +163,26 @@ def internal_fixtures():
     print("internal vectors", len(out), "arrays")
 
 
+# five sites of the 5-degree grid at the benchmark's resolution (config 2: IGRF-14 2020
+# + T96 quiet, 0.01..20 GV in 0.01 GV steps, nk1 400, 1e6 steps), geodetic latitude and
+# longitude -> the reference's geocentric input (WGS84, 20 km)
+GRID5_SITES = [(60.0, 0.0), (40.0, 120.0), (0.0, 90.0), (-30.0, 300.0), (-60.0, 150.0)]
+
+
+def grid5_fixtures():
+    """Reference runs of five grid sites at the benchmark's resolution."""
+    import cor_amd as A
+    for lat, lon in GRID5_SITES:
+        r0, latc = A.geodetic_to_geocentric(lat, 20.0)
+        name = f"grid5_{int(lat):+03d}_{int(lon):03d}"
+        inp = HERE / f"{name}.in"
+        L.write_cutoff_input(inp, 0.01, -1.0, 20.0, r0, latc, lon, latc, lon, nk1=400, dl=0.01,
+                             ext_params=[-20, 2, 0, -2])
+        txt = L.run_ref_cutoff(inp, internal="igrf14", external="t96", steps=1000000)
+        (HERE / f"{name}.out").write_text(txt)
+        print(name, txt.strip().splitlines()[-1])
+
+
 def field_fixtures():
     """Field-map runs of the reference CLI (-m field, src/field.c:6-54), input and output
     text: IGRF-14 + T96 on a 20 x 30 degree grid at 1.5 Re, and IGRF-14 + the
@@ -222,6 +244,9 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "internal":
         internal_fixtures()
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "grid5":
+        grid5_fixtures()
+        return
     ref = L.RefLib()
     rng = np.random.default_rng(20240501)
     out = {}
@@ -266,6 +291,7 @@ def main():
     field_fixtures()
     track_fixtures()
     internal_fixtures()
+    grid5_fixtures()
 
 
 if __name__ == "__main__":
