@@ -260,6 +260,29 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
     }
 }
 
+// Device memory of cor_gpu_trace_noalloc: the epoch block, the scan descriptors, the
+// queue head, the run counters and the per-scan step sums of one call, in a file-scope
+// array that every GPU holds once for the life of the process.  Host code uses the
+// device address each GPU reports for it (noalloc_scratch[device], taken in
+// cor_gpu_init), never the array's own name, which is a host-side stand-in.
+struct NoAllocScratch {
+    EpochDev epoch;
+    ScanDev scans[COR_NOALLOC_MAX_SCANS];
+    ScanStat stat[COR_NOALLOC_MAX_SCANS];
+    unsigned long long head;
+    RunStats run;
+};
+static __device__ __align__(256) unsigned char scratch[sizeof(NoAllocScratch)];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+NoAllocScratch *g_noalloc[NOALLOC_MAX_DEVICES] = {};
+// host-side staging of the same blocks, one per device (static storage: the call copies
+// from it and returns only when the copies are done)
+struct NoAllocHost {
+    EpochDev epoch;
+    ScanDev scans[COR_NOALLOC_MAX_SCANS];
+};
+NoAllocHost g_noalloc_host[NOALLOC_MAX_DEVICES];
+
 // Accepted per-lane steps before a trajectory moves to the cooperative mode.  On the
 // 5-degree IGRF + T96 grid the median trajectory runs 435 steps, 0.8 % run past 65536 and
 // hold 40 % of all steps; about 1 in 3000 reaches the 10^6-step limit.  A per-lane step
@@ -341,6 +364,12 @@ int cor_gpu_init(int device) {
     HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
     g_wall_khz = khz > 0 ? (double)khz : 0.;
     if (!g_stream) HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // this device's copy of the allocation-free entry point's scratch array
+    if (device < NOALLOC_MAX_DEVICES && !g_noalloc[device]) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_noalloc[device] = (NoAllocScratch *)ptr;
+    }
     g_device = device;
     return COR_OK;
 }
@@ -764,6 +793,98 @@ int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const 
     if (!rc) rc = cor_trace_plan_results(p, traj, cut);
     cor_trace_plan_destroy(p);
     return rc;
+}
+
+int cor_gpu_trace_noalloc(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                          int32_t n_scans, const cor_trace_opts *opts, void *d_traj, cor_cutoff *d_cut,
+                          void *stream) {
+    if (!epoch || !opts || !scans || !d_traj || !d_cut) return COR_E_INVALID;
+    if (n_scans < 1 || n_scans > COR_NOALLOC_MAX_SCANS) {
+        snprintf(g_hip_error, sizeof g_hip_error, "cor_gpu_trace_noalloc takes 1 to %d scans, not %d",
+                 COR_NOALLOC_MAX_SCANS, (int)n_scans);
+        return COR_E_INVALID;
+    }
+    if (opts->precision != COR_PREC_FP64 && opts->precision != COR_PREC_MIXED) return COR_E_INVALID;
+    if (!(opts->particle_mass > 0.) || !(opts->tu_angle > 0.)) return COR_E_INVALID;
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (g_device >= NOALLOC_MAX_DEVICES || !g_noalloc[g_device]) return COR_E_NO_DEVICE;
+    NoAllocScratch *sc = g_noalloc[g_device];
+    int intk, ext, cru;
+    if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
+    if (cru != CRU_NONE) {
+        snprintf(g_hip_error, sizeof g_hip_error, "cor_gpu_trace_noalloc: crustal tables are not held in its scratch");
+        return COR_E_UNSUPPORTED;
+    }
+    const void *kern = opts->precision == COR_PREC_FP64 ? trace_kernel_fp64(intk, ext, cru)
+                                                       : trace_kernel_mixed(intk, ext, cru);
+    if (!kern) return COR_E_UNSUPPORTED;
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    EpochDev &h_epoch = g_noalloc_host[g_device].epoch;
+    ScanDev *h_scans = g_noalloc_host[g_device].scans;
+    int64_t total = 0;
+    for (int32_t k = 0; k < n_scans; k++) {
+        const int64_t n = cor_scan_count(&scans[k]);
+        if (n < 0 || scans[k].nk1 <= 0 || !isfinite(scans[k].zn) || scans[k].zn == 0.) return COR_E_INVALID;
+        build_scan(scans[k], total, (int32_t)n, h_scans[k]);
+        total += n;
+    }
+    if (total >= (int64_t)0x7fffffffll) return COR_E_INVALID;
+    build_epoch(models, epoch, h_epoch);
+    HIP_TRY(hipMemcpyAsync(&sc->epoch, &h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc->scans, h_scans, sizeof(ScanDev) * n_scans, hipMemcpyHostToDevice, s));
+    if (needs_coop_tables(opts->precision, ext)) HIP_TRY(build_coop_tables(&sc->epoch, s));
+    // the caller's record block: status, steps, attempts, handover (4 bytes each), then
+    // the eight doubles, each an n-strided array
+    const size_t n = (size_t)total;
+    unsigned char *base = (unsigned char *)d_traj;
+    TraceArgs a;
+    memset(&a, 0, sizeof a);
+    a.out.status = (int32_t *)base;
+    a.out.steps = (uint32_t *)(base + 4 * n);
+    a.out.attempts = (uint32_t *)(base + 8 * n);
+    a.out.handover = (uint32_t *)(base + 12 * n);
+    a.out.vc = (double *)(base + 16 * n);
+    a.out.r = a.out.vc + n;
+    a.out.td = a.out.r + n;
+    a.out.fei = a.out.td + n;
+    a.out.ast = a.out.fei + n;
+    a.out.asf = a.out.ast + n;
+    a.out.time = a.out.asf + n;
+    a.out.length = a.out.time + n;
+    a.epoch = &sc->epoch;
+    a.scans = sc->scans;
+    a.n_scans = n_scans;
+    a.order = nullptr;                     // rigidity order: the results do not depend on it
+    a.n_items = total;
+    a.head = &sc->head;
+    a.step_limit = opts->step_limit;
+    a.tu_angle = opts->tu_angle;
+    a.cos_tu = cos(opts->tu_angle);
+    a.mass = opts->particle_mass;
+    a.inv_mass = 1.0 / opts->particle_mass;
+    a.coop_on = 0;                         // per-lane mode: its park slots would scale with the batch
+    a.stats = &sc->run;
+    // zero the queue head and the run counters (a kernel, also under graph capture)
+    reset_head_kernel<<<1, 1, 0, s>>>(&sc->head, nullptr, &sc->run);
+    HIP_TRY(hipGetLastError());
+    if (total > 0) {
+        int per_cu = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+        const int64_t need = (total + 63) / 64;
+        int64_t grid = (int64_t)std::max(per_cu, 1) * g_num_cus;
+        int block = 256;
+        if (need * 64 < (int64_t)block * grid) {
+            block = 64;
+            grid = std::min<int64_t>(need, (int64_t)std::max(per_cu, 1) * 4 * g_num_cus);
+        }
+        void *args[] = {&a};
+        HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(block), args, 0, s));
+    }
+    penumbra_kernel<<<n_scans, 64, 0, s>>>(sc->scans, a.out.status, a.out.steps, a.out.attempts, d_cut, sc->stat);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return COR_OK;
 }
 
 int cor_gpu_trace_tracks(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,

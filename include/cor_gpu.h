@@ -185,6 +185,28 @@ int cor_gpu_trace_series(const cor_model_set *models, const ModelsParams *epochs
                          const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
                          int32_t in_flight, cor_traj *traj, cor_cutoff *cut);
 
+/* cor_gpu_trace for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  No counterpart in
+ * the reference.  Nothing is allocated: the epoch block, the scan descriptors and the
+ * run's counters live in a fixed scratch array of about 0.5 MB that each GPU holds for
+ * the life of the process; the caller passes device buffers for the results:
+ *   d_traj  cor_scan_count summed over the scans (n) x COR_TRAJ_SOA_BYTES bytes, 8-byte
+ *           aligned: int32 status[n], uint32 steps[n], attempts[n], handover[n], then
+ *           double vc[n], r[n], td[n], fei[n], ast[n], asf[n], time[n], length_m[n]
+ *           (the fields of cor_traj, one array each);
+ *   d_cut   n_scans cor_cutoff.
+ * At most COR_NOALLOC_MAX_SCANS scans (COR_E_INVALID beyond); no crustal model
+ * (COR_E_UNSUPPORTED).  Always the per-lane mode, whose results equal cor_gpu_trace's
+ * with opts->coop < 0 bit for bit; opts->queue_order is ignored (it changes no result).
+ * Runs on `stream` (NULL = the library's stream) and returns when the results are in
+ * place.  Every call on a GPU uses that GPU's one scratch array: calls that overlap on
+ * one GPU race. */
+#define COR_NOALLOC_MAX_SCANS 4096
+#define COR_TRAJ_SOA_BYTES 80
+int cor_gpu_trace_noalloc(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                          int32_t n_scans, const cor_trace_opts *opts, void *d_traj, cor_cutoff *d_cut,
+                          void *stream);
+
 /* Per-step tracks of every trajectory of the scans in FP64, as the reference's
  * trajectory mode writes them (trajectory_simulation, src/trajectory.c:163-273): one
  * point per accepted step that does not end the trajectory.  counts (nullable) receives
