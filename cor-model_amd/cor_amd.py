@@ -18,7 +18,7 @@ DP = C.POINTER(C.c_double)
 
 COR_OK, COR_E_INVALID, COR_E_NO_DEVICE, COR_E_HIP, COR_E_UNSUPPORTED, COR_E_NOMEM = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_MIXED, PREC_MIXED_WAVE, PREC_MIXED_GROUP, PREC_MIXED_GROUP16, PREC_MIXED_GROUP32 = 0, 1, 2, 3, 4, 5
-PREC_FP64_WAVE, PREC_FP64_GROUP = 6, 7
+PREC_FP64_WAVE, PREC_FP64_GROUP, PREC_MIXED_GROUP4 = 6, 7, 8
 INTERNAL_NONE, INTERNAL_IGRF = 0, 1
 EXTERNAL_NONE, EXTERNAL_T96, EXTERNAL_TS05 = 0, 1, 2
 CRUSTAL_NONE, CRUSTAL_SH = 0, 1
@@ -40,7 +40,8 @@ EXPORTED = [
     "cor_scan_count", "cor_geodetic_to_geocentric", "cor_direction_to_geo", "cor_gpu_synchronize",
     "cor_trace_plan_set_epoch", "cor_gpu_trace_series", "cor_refine_scan", "cor_gpu_trace_tracks",
     "trajectory_simulation", "calculate_position_and_speed", "string_to_uint64", "string_to_double",
-    "cor_gpu_rk_step", "IGRF_9_COEFS", "IGRF_10_COEFS", "IGRF_11_COEFS", "IGRF_12_COEFS",
+    "cor_gpu_rk_step", "cor_trace_plan_create_epochs", "cor_gpu_trace_epochs",
+    "cor_trace_autotune", "cor_trace_plan_progress", "IGRF_9_COEFS", "IGRF_10_COEFS", "IGRF_11_COEFS", "IGRF_12_COEFS",
     "IGRF_13_COEFS", "IGRF_14_COEFS", "HIST_COEFS", "CALS10K_2_COEFS",
 ]
 
@@ -80,7 +81,7 @@ class Scan(C.Structure):
 class TraceOpts(C.Structure):
     _fields_ = [("step_limit", C.c_uint64), ("tu_angle", C.c_double), ("particle_mass", C.c_double),
                 ("precision", C.c_int32), ("queue_order", C.c_int32), ("coop", C.c_int32),
-                ("handover_steps", C.c_int32)]
+                ("handover_steps", C.c_int32), ("waves_per_simd", C.c_int32), ("block_threads", C.c_int32)]
 
 
 class Cutoff(C.Structure):
@@ -136,6 +137,7 @@ def lib():
                                         C.c_int32, C.POINTER(TraceOpts), C.POINTER(C.c_void_p)]
     L.cor_trace_plan_run.argtypes = [C.c_void_p, C.c_void_p]
     L.cor_trace_plan_sync.argtypes = [C.c_void_p]
+    L.cor_trace_plan_progress.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.cor_trace_plan_count.argtypes = [C.c_void_p]
     L.cor_trace_plan_count.restype = C.c_int64
     L.cor_trace_plan_results.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -147,6 +149,13 @@ def lib():
     L.cor_trace_plan_set_epoch.argtypes = [C.c_void_p, C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p]
     L.cor_gpu_trace_series.argtypes = [C.POINTER(ModelSet), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                        C.POINTER(TraceOpts), C.c_int32, C.c_void_p, C.c_void_p]
+    L.cor_trace_autotune.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p, C.c_int32,
+                                     C.POINTER(TraceOpts), C.c_int32, C.c_uint64, C.POINTER(TraceOpts),
+                                     C.POINTER(C.c_double)]
+    L.cor_trace_plan_create_epochs.argtypes = [C.POINTER(ModelSet), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                               C.POINTER(TraceOpts), C.c_void_p]
+    L.cor_gpu_trace_epochs.argtypes = [C.POINTER(ModelSet), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                       C.POINTER(TraceOpts), C.c_void_p, C.c_void_p]
     L.cor_gpu_trace_tracks.argtypes = [C.POINTER(ModelSet), C.POINTER(ModelsParams), C.c_void_p, C.c_int32,
                                        C.POINTER(TraceOpts), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p,
                                        C.c_int64]
@@ -223,16 +232,29 @@ def scan_count(row):
 
 
 class Plan:
-    """cor_trace_plan: epoch + scans uploaded to HBM; run() traces them all."""
+    """cor_trace_plan: epoch + scans uploaded to HBM; run() traces them all.  With a list
+    of epochs (params as a list of ModelsParams) it is the multi-epoch plan
+    (cor_trace_plan_create_epochs): one launch over every (epoch, scan) pair, whose
+    results() come back epoch-major as (n_epochs, count) and (n_epochs, n_scans)."""
 
     def __init__(self, models, params, scans, step_limit=1000000, tu_angle=0.01, mass=M_PROTON,
-                 precision=PREC_FP64, queue_order=0, coop=0, handover_steps=0):
+                 precision=PREC_FP64, queue_order=0, coop=0, handover_steps=0, waves_per_simd=0, block_threads=0):
         self.scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
-        opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, coop, handover_steps)
+        opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, coop, handover_steps, waves_per_simd,
+                         block_threads)
         h = C.c_void_p()
         self._models = models
-        check(lib().cor_trace_plan_create(C.byref(models), C.byref(params), self.scans.ctypes.data,
-                                          len(self.scans), C.byref(opts), C.byref(h)), "plan_create")
+        self.multi = isinstance(params, (list, tuple))
+        if self.multi:
+            arr = (ModelsParams * len(params))(*params)
+            self.n_epochs = len(params)
+            check(lib().cor_trace_plan_create_epochs(C.byref(models), C.cast(arr, C.c_void_p), len(params),
+                                                     self.scans.ctypes.data, len(self.scans), C.byref(opts),
+                                                     C.byref(h)), "plan_create_epochs")
+        else:
+            self.n_epochs = 1
+            check(lib().cor_trace_plan_create(C.byref(models), C.byref(params), self.scans.ctypes.data,
+                                              len(self.scans), C.byref(opts), C.byref(h)), "plan_create")
         self.h = h
         self.n = lib().cor_trace_plan_count(h)
 
@@ -242,16 +264,25 @@ class Plan:
     def sync(self):
         check(lib().cor_trace_plan_sync(self.h), "plan_sync")
 
+    def progress(self):
+        """(trajectories ended so far, run complete) of the last run, without waiting."""
+        f, d = C.c_int64(), C.c_int32()
+        check(lib().cor_trace_plan_progress(self.h, C.byref(f), C.byref(d)), "plan_progress")
+        return f.value, bool(d.value)
+
     def set_epoch(self, params, stream=None):
         """Re-point the plan at another epoch (same model kinds) before the next run."""
         check(lib().cor_trace_plan_set_epoch(self.h, C.byref(self._models), C.byref(params),
                                              C.c_void_p(stream) if stream else None), "plan_set_epoch")
 
     def results(self, traj=True):
-        cut = np.zeros(len(self.scans), dtype=CUT_DTYPE)
+        cut = np.zeros(len(self.scans) * self.n_epochs, dtype=CUT_DTYPE)
         tr = np.zeros(self.n, dtype=TRAJ_DTYPE) if traj else None
         check(lib().cor_trace_plan_results(self.h, tr.ctypes.data if traj else None, cut.ctypes.data),
               "plan_results")
+        if self.multi:
+            cut = cut.reshape(self.n_epochs, len(self.scans))
+            tr = tr.reshape(self.n_epochs, -1) if traj else None
         return tr, cut
 
     def stats(self):
@@ -294,6 +325,37 @@ def trace_series(models, epochs, scans, in_flight=0, step_limit=1000000, tu_angl
                                      len(scans), C.byref(opts), in_flight, tr.ctypes.data if traj else None,
                                      cut.ctypes.data), "cor_gpu_trace_series")
     return tr, cut
+
+
+def trace_epochs(models, epochs, scans, step_limit=1000000, tu_angle=0.01, mass=M_PROTON, precision=PREC_FP64,
+                 queue_order=0, traj=True):
+    """cor_gpu_trace_epochs: the scans at every epoch in one launch; results shaped
+    (n_epochs, count) and (n_epochs, n_scans)."""
+    scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
+    count = sum(scan_count(r) for r in scans)
+    arr = (ModelsParams * len(epochs))(*epochs)
+    opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, 0, 0)
+    cut = np.zeros((len(epochs), len(scans)), dtype=CUT_DTYPE)
+    tr = np.zeros((len(epochs), count), dtype=TRAJ_DTYPE) if traj else None
+    check(lib().cor_gpu_trace_epochs(C.byref(models), C.cast(arr, C.c_void_p), len(epochs), scans.ctypes.data,
+                                     len(scans), C.byref(opts), tr.ctypes.data if traj else None,
+                                     cut.ctypes.data), "cor_gpu_trace_epochs")
+    return tr, cut
+
+
+def autotune(models, params, scans, precision=PREC_MIXED, step_limit=1000000, tu_angle=0.01, mass=M_PROTON,
+             coop=0, handover_steps=0, sample_scans=256, seed=0):
+    """cor_trace_autotune: the launch parameters (waves per SIMD, threads per block, queue
+    order) that trace a seeded sample of the scans fastest; returned as Plan keywords
+    plus the sample's tracer time in ms."""
+    scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
+    base = TraceOpts(step_limit, tu_angle, mass, precision, 1, coop, handover_steps, 0, 0)
+    best = TraceOpts()
+    ms = C.c_double()
+    check(lib().cor_trace_autotune(C.byref(models), C.byref(params), scans.ctypes.data, len(scans), C.byref(base),
+                                   sample_scans, seed, C.byref(best), C.byref(ms)), "cor_trace_autotune")
+    return {"waves_per_simd": best.waves_per_simd, "block_threads": best.block_threads,
+            "queue_order": best.queue_order, "sample_ms": ms.value}
 
 
 def trace_tracks(models, params, scans, step_limit=1000000, tu_angle=0.01, mass=M_PROTON):

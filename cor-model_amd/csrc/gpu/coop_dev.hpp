@@ -323,14 +323,14 @@ __device__ __forceinline__ void wave_sum(F *p) {
     for (int k = 0; k < N; k++) p[k] = readlane_any(p[k], 63);
 }
 
-// Sum of each of N values over a group of G lanes (G = 8: the three DPP steps within a
-// half row; G = 16: the four row steps; G = 32: the row sums plus the partner row's,
-// read by a lane permute; every lane of the group ends with the same bits, since each
-// step adds two values that the partner lane adds in the other order).  G = 64:
-// wave_sum.
+// Sum of each of N values over a group of G lanes (G = 4: the two DPP steps within a
+// quad; G = 8: three steps within a half row; G = 16: the four row steps; G = 32: the
+// row sums plus the partner row's, read by a lane permute; every lane of the group ends
+// with the same bits, since each step adds two values that the partner lane adds in the
+// other order).  G = 64: wave_sum.
 template <int N, int G, class F>
 __device__ __forceinline__ void group_sum(F *p) {
-    static_assert(G == 8 || G == 16 || G == 32 || G == 64, "groups of 8, 16, 32 or 64 lanes");
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "groups of 4, 8, 16, 32 or 64 lanes");
     if constexpr (G == 64) {
         wave_sum<N>(p);
     } else {
@@ -338,8 +338,10 @@ __device__ __forceinline__ void group_sum(F *p) {
         for (int k = 0; k < N; k++) p[k] = dpp_add<0xB1, 0xF>(p[k]);    // quad_perm [1,0,3,2]
 #pragma unroll
         for (int k = 0; k < N; k++) p[k] = dpp_add<0x4E, 0xF>(p[k]);    // quad_perm [2,3,0,1]
+        if constexpr (G >= 8) {
 #pragma unroll
-        for (int k = 0; k < N; k++) p[k] = dpp_add<0x141, 0xF>(p[k]);   // row_half_mirror
+            for (int k = 0; k < N; k++) p[k] = dpp_add<0x141, 0xF>(p[k]);   // row_half_mirror
+        }
         if constexpr (G >= 16) {
 #pragma unroll
             for (int k = 0; k < N; k++) p[k] = dpp_add<0x140, 0xF>(p[k]);   // row_mirror
@@ -975,7 +977,7 @@ template <class F, int INT, int EXT, int GS>
 __device__ __forceinline__ v3<F> coop_field(const F *G, const F *H, const F *REC, const F *Af,
                                             const T96Params<F> &P, const CoopTabT<F> &T, v3<double> geo_m,
                                             v3<double> gsm_m, int gl) {
-    static_assert(GS == 8 || GS == 16 || GS == 32 || GS == 64, "groups of 8, 16, 32 or 64 lanes");
+    static_assert(GS == 4 || GS == 8 || GS == 16 || GS == 32 || GS == 64, "groups of 4, 8, 16, 32 or 64 lanes");
     IgrfShare<F> I;
     T96Share<F> X;
     F p[6] = {0, 0, 0, 0, 0, 0};

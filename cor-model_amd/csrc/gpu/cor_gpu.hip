@@ -157,6 +157,20 @@ hipError_t upload_sh(const cor_model_set *m, void **buf, size_t *cap, EpochDev &
     return e;
 }
 
+// The SH table pointers of an epoch block built on the host, for tables already on the
+// device (the model set of a series is fixed, so its tables are uploaded once).
+void point_sh(const cor_model_set *m, void *d_sh, EpochDev &E) {
+    if (m->crustal != COR_CRUSTAL_SH || !d_sh) return;
+    const size_t nn = (size_t)m->sh_nm_max * (size_t)m->sh_nm_max;
+    double *dd = (double *)d_sh;
+    float *ff = (float *)(dd + 2 * nn);
+    E.sh_nm = m->sh_nm_max;
+    E.shG = dd;
+    E.shH = dd + nn;
+    E.shGf = ff;
+    E.shHf = ff + nn;
+}
+
 // Station position and arrival-direction trig as src/trajectory.c:18-31 and
 // src/utils.c:28-41 compute them (host libm, like the reference).
 void build_scan(const cor_scan &s, int64_t first, int32_t n, ScanDev &d) {
@@ -180,6 +194,8 @@ void build_scan(const cor_scan &s, int64_t first, int32_t n, ScanDev &d) {
     d.nk1 = s.nk1;
     d.n = n;
     d.first = first;
+    d.epoch = 0;
+    d.pad = 0;
 }
 
 // Per scan: src/trajectory.c:49-54,114-115,133-145,156-160.  rmx1/rmx2/rmi are the
@@ -249,6 +265,7 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
     st->t_start = ~0ull;
     st->t_drain = 0ull;
     st->t_lane_end = 0ull;
+    st->finished = 0ull;
     if (coop) {
         coop->res = 0ull;
         coop->head = 0ull;
@@ -257,6 +274,7 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
         coop->done = 0ull;
         coop->converting = 0;
         coop->feeders = 0;
+        coop->run = coop->run + 1u;   // this run's id for the park records
     }
 }
 
@@ -310,6 +328,7 @@ hipError_t build_coop_tables(EpochDev *d_epoch, hipStream_t s) {
 
 struct cor_trace_plan {
     int intk, ext, cru, prec;
+    int32_t n_epochs;            // > 1: the multi-epoch tracer (d_epoch holds that many blocks)
     const void *kernel;
     EpochDev *d_epoch;
     ScanDev *d_scans;
@@ -327,7 +346,6 @@ struct cor_trace_plan {
     int coop_on;                 // 0: per-lane mode only
     uint32_t handover_steps;     // per-lane accepted steps before the cooperative mode
     size_t park_slots;
-    uint32_t run_id;
     CoopCtl *d_coop;
     ParkDev *d_park;
     RunStats *d_run;
@@ -340,6 +358,15 @@ struct cor_trace_plan {
     EpochDev *h_epoch;           // pinned staging of set_epoch uploads
     hipEvent_t ev_up;            // the last set_epoch upload has left h_epoch
     bool up_pending;
+    // the epoch graph (cor_gpu_trace_series): upload of h_epoch, lane tables, reset,
+    // trace, penumbra and the copies of every result into h_res, captured once and
+    // replayed per epoch
+    hipStream_t s_progress;      // cor_trace_plan_progress reads the run's counter on it
+    unsigned long long *h_progress;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    unsigned char *h_res;        // pinned: CoopCtl, cut[n_scans], then the trajectory arrays
+    hipEvent_t ev_graph;         // the last replay is complete
 };
 
 extern "C" {
@@ -426,30 +453,61 @@ void cor_trace_plan_destroy(cor_trace_plan *p) {
     (void)hipFree(p->d_trk_off);
     for (int k = 0; k < 3; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
+    if (p->ev_graph) {
+        (void)hipEventSynchronize(p->ev_graph);
+        (void)hipEventDestroy(p->ev_graph);
+    }
+    if (p->s_progress) {
+        (void)hipStreamSynchronize(p->s_progress);
+        (void)hipStreamDestroy(p->s_progress);
+    }
+    if (p->h_progress) (void)hipHostFree(p->h_progress);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->h_res) (void)hipHostFree(p->h_res);
     if (p->up_pending) (void)hipEventSynchronize(p->ev_up);
     if (p->ev_up) (void)hipEventDestroy(p->ev_up);
     if (p->h_epoch) (void)hipHostFree(p->h_epoch);
     delete p;
 }
 
-int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
-                          int32_t n_scans, const cor_trace_opts *opts, cor_trace_plan **out) {
+}  // extern "C"
+
+namespace {
+// A plan over n_epochs epochs of the same scans.  One epoch: the tracer kernels with the
+// epoch staged per workgroup.  Several: the multi-epoch tracer, whose scan list is every
+// (epoch, scan) pair, epoch-major, each naming its epoch block.
+int plan_create_impl(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
+                     const cor_scan *scans, int32_t n_scans_in, const cor_trace_opts *opts, cor_trace_plan **out) {
     if (!out) return COR_E_INVALID;
     *out = nullptr;
-    if (!epoch || !opts || n_scans < 1 || !scans) return COR_E_INVALID;
+    if (!epochs || n_epochs < 1 || !opts || n_scans_in < 1 || !scans) return COR_E_INVALID;
     if (opts->precision != COR_PREC_FP64 && opts->precision != COR_PREC_MIXED) return COR_E_INVALID;
     if (!(opts->particle_mass > 0.) || !(opts->tu_angle > 0.)) return COR_E_INVALID;
+    if ((int64_t)n_epochs * n_scans_in > 0x7fffffffll) return COR_E_INVALID;
+    if (opts->waves_per_simd < 0 || (opts->block_threads != 0 && opts->block_threads != 64 &&
+                                     opts->block_threads != 256))
+        return COR_E_INVALID;
     int rc = ensure_init();
     if (rc) return rc;
     int intk, ext, cru;
     if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
+    const bool multi = n_epochs > 1;
     // FP64 with the cooperative tail (opt-in) has a kernel of its own; model sets the
     // cooperative evaluator does not cover run per-lane
-    const bool fp64_coop = opts->precision == COR_PREC_FP64 && opts->coop > 0 && trace_kernel_fp64_coop(intk, ext, cru);
-    const void *kern = opts->precision == COR_PREC_FP64 ? (fp64_coop ? trace_kernel_fp64_coop(intk, ext, cru)
-                                                                     : trace_kernel_fp64(intk, ext, cru))
-                                                       : trace_kernel_mixed(intk, ext, cru);
+    const bool fp64_coop = !multi && opts->precision == COR_PREC_FP64 && opts->coop > 0 &&
+                           trace_kernel_fp64_coop(intk, ext, cru);
+    const void *kern;
+    if (multi) {
+        kern = opts->precision == COR_PREC_FP64 ? trace_kernel_epochs_fp64(intk, ext, cru)
+                                                : trace_kernel_epochs_mixed(intk, ext, cru);
+    } else {
+        kern = opts->precision == COR_PREC_FP64 ? (fp64_coop ? trace_kernel_fp64_coop(intk, ext, cru)
+                                                             : trace_kernel_fp64(intk, ext, cru))
+                                                : trace_kernel_mixed(intk, ext, cru);
+    }
     if (!kern) return COR_E_UNSUPPORTED;
+    const int32_t n_scans = n_epochs * n_scans_in;
 
     cor_trace_plan *p = new cor_trace_plan();   // value-initialised: every pointer null
     p->intk = intk;
@@ -458,20 +516,25 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     p->prec = opts->precision;
     p->kernel = kern;
     p->n_scans = n_scans;
+    p->n_epochs = n_epochs;
     p->step_limit = opts->step_limit;
     p->tu_angle = opts->tu_angle;
     p->cos_tu = cos(opts->tu_angle);
     p->mass = opts->particle_mass;
     p->h_scans.resize(n_scans);
     int64_t total = 0;
-    for (int32_t s = 0; s < n_scans; s++) {
-        const int64_t n = cor_scan_count(&scans[s]);
-        if (n < 0 || scans[s].nk1 <= 0 || !isfinite(scans[s].zn) || scans[s].zn == 0.) {
-            delete p;
-            return COR_E_INVALID;
+    for (int32_t e = 0; e < n_epochs; e++) {
+        for (int32_t s = 0; s < n_scans_in; s++) {
+            const int64_t n = cor_scan_count(&scans[s]);
+            if (n < 0 || scans[s].nk1 <= 0 || !isfinite(scans[s].zn) || scans[s].zn == 0.) {
+                delete p;
+                return COR_E_INVALID;
+            }
+            ScanDev &d = p->h_scans[(size_t)e * n_scans_in + s];
+            build_scan(scans[s], total, (int32_t)n, d);
+            d.epoch = e;
+            total += n;
         }
-        build_scan(scans[s], total, (int32_t)n, p->h_scans[s]);
-        total += n;
     }
     if (total >= (int64_t)0xffffffffll) {
         delete p;
@@ -479,15 +542,16 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     }
     p->n_items = total;
 
-    EpochDev E;
-    build_epoch(models, epoch, E);
+    std::vector<EpochDev> E(n_epochs);
+    for (int32_t e = 0; e < n_epochs; e++) build_epoch(models, &epochs[e], E[e]);
     const size_t nt = (size_t)std::max<int64_t>(total, 1);
     {
-        const hipError_t e = upload_sh(models, &p->d_sh, &p->sh_cap, E);
+        const hipError_t e = upload_sh(models, &p->d_sh, &p->sh_cap, E[0]);
         if (e != hipSuccess) {
             cor_trace_plan_destroy(p);
             return fail_hip(e, "upload_sh");
         }
+        for (int32_t k = 1; k < n_epochs; k++) point_sh(models, p->d_sh, E[k]);
     }
 #define PLAN_TRY(call)                                    \
     do {                                                  \
@@ -497,8 +561,8 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
             return fail_hip(e_, #call);                   \
         }                                                 \
     } while (0)
-    PLAN_TRY(hipMalloc(&p->d_epoch, sizeof(EpochDev)));
-    PLAN_TRY(hipMemcpy(p->d_epoch, &E, sizeof(EpochDev), hipMemcpyHostToDevice));
+    PLAN_TRY(hipMalloc(&p->d_epoch, sizeof(EpochDev) * n_epochs));
+    PLAN_TRY(hipMemcpy(p->d_epoch, E.data(), sizeof(EpochDev) * n_epochs, hipMemcpyHostToDevice));
     PLAN_TRY(hipMalloc(&p->d_scans, sizeof(ScanDev) * n_scans));
     PLAN_TRY(hipMemcpy(p->d_scans, p->h_scans.data(), sizeof(ScanDev) * n_scans, hipMemcpyHostToDevice));
     PLAN_TRY(hipMalloc(&p->d_head, sizeof(unsigned long long)));
@@ -535,23 +599,29 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
         PLAN_TRY(hipMemcpy(p->d_order, order.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice));
     }
 
-    // the cooperative evaluator covers IGRF and T96 (TS05 and SH traces stay per-lane)
-    p->coop_on = (p->prec == COR_PREC_MIXED ? opts->coop >= 0 : opts->coop > 0) && ext != EXT_TS05 && cru == CRU_NONE;
+    // the cooperative evaluator covers IGRF and T96 (TS05 and SH traces stay per-lane, and
+    // so does the multi-epoch tracer)
+    p->coop_on = !multi && (p->prec == COR_PREC_MIXED ? opts->coop >= 0 : opts->coop > 0) && ext != EXT_TS05 &&
+                 cru == CRU_NONE;
     p->handover_steps = opts->handover_steps > 0 ? (uint32_t)opts->handover_steps : COOP_HANDOVER_DEFAULT;
 
     // persistent grid: as many resident blocks as the occupancy allows, but no more
     // blocks than are needed to give every trajectory a lane at the start (or, with the
     // cooperative tail, a wavefront to finish in)
+    // (a 256-thread block is four wavefronts, one per SIMD, so blocks per CU = resident
+    // wavefronts per SIMD; the caller may ask for fewer, and for a block size)
     p->block = 256;
     int per_cu = 0;
     PLAN_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p->block, 0));
     if (per_cu < 1) per_cu = 1;
+    if (opts->waves_per_simd > 0) per_cu = std::min(per_cu, (int)opts->waves_per_simd);
     int64_t grid = (int64_t)per_cu * g_num_cus;
     const int64_t need = (total + 63) / 64;   // one wavefront per 64 trajectories
-    if (need * 64 < (int64_t)p->block * grid) {
-        // small launches: single-wavefront blocks spread over all CUs
+    if (opts->block_threads == 64 || (opts->block_threads == 0 && need * 64 < (int64_t)p->block * grid)) {
+        // single-wavefront blocks spread over all CUs (by default for small launches)
         p->block = 64;
-        grid = std::min<int64_t>(p->coop_on ? total : need, (int64_t)per_cu * 4 * g_num_cus);
+        grid = (int64_t)per_cu * 4 * g_num_cus;
+        if (need * 64 < (int64_t)p->block * grid) grid = std::min<int64_t>(p->coop_on ? total : need, grid);
     }
     p->grid = (int)std::max<int64_t>(grid, 1);
     if (p->coop_on || needs_coop_tables(p->prec, ext)) {
@@ -566,6 +636,7 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
         p->park_slots = (size_t)total + (size_t)p->grid * p->block;
         PLAN_TRY(hipMalloc(&p->d_coop, sizeof(CoopCtl)));
         PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * p->park_slots));
+        PLAN_TRY(hipMemsetAsync(p->d_coop, 0, sizeof(CoopCtl), g_stream));
         PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * p->park_slots, g_stream));
         PLAN_TRY(hipStreamSynchronize(g_stream));
     }
@@ -574,12 +645,103 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
     *out = p;
     return COR_OK;
 }
+}  // namespace
+
+extern "C" {
+
+int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                          int32_t n_scans, const cor_trace_opts *opts, cor_trace_plan **out) {
+    return plan_create_impl(models, epoch, 1, scans, n_scans, opts, out);
+}
+
+int cor_trace_plan_create_epochs(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
+                                 const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
+                                 cor_trace_plan **out) {
+    return plan_create_impl(models, epochs, n_epochs, scans, n_scans, opts, out);
+}
+
+int cor_gpu_trace_epochs(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
+                         const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts, cor_traj *traj,
+                         cor_cutoff *cut) {
+    cor_trace_plan *p = nullptr;
+    int rc = plan_create_impl(models, epochs, n_epochs, scans, n_scans, opts, &p);
+    if (rc) return rc;
+    rc = cor_trace_plan_run(p, nullptr);
+    if (!rc) rc = cor_trace_plan_results(p, traj, cut);
+    cor_trace_plan_destroy(p);
+    return rc;
+}
+
+int cor_trace_autotune(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                       int32_t n_scans, const cor_trace_opts *base, int32_t sample_scans, uint64_t seed,
+                       cor_trace_opts *best, double *best_ms) {
+    if (!models || !epoch || !scans || n_scans < 1 || !base || !best || sample_scans < 1) return COR_E_INVALID;
+    int rc = ensure_init();
+    if (rc) return rc;
+    int intk, ext, cru;
+    if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
+    // the sample: a seeded choice of scans (partial Fisher-Yates with a 64-bit LCG)
+    std::vector<int32_t> idx(n_scans);
+    for (int32_t k = 0; k < n_scans; k++) idx[k] = k;
+    const int32_t ns = std::min(sample_scans, n_scans);
+    uint64_t x = seed * 6364136223846793005ull + 1442695040888963407ull;
+    for (int32_t k = 0; k < ns; k++) {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        const int32_t j = k + (int32_t)((x >> 33) % (uint64_t)(n_scans - k));
+        std::swap(idx[k], idx[j]);
+    }
+    std::sort(idx.begin(), idx.begin() + ns);
+    std::vector<cor_scan> sample(ns);
+    for (int32_t k = 0; k < ns; k++) sample[k] = scans[idx[k]];
+    // the register limit of this workload's kernel
+    const void *kern = base->precision == COR_PREC_FP64 ? trace_kernel_fp64(intk, ext, cru)
+                                                        : trace_kernel_mixed(intk, ext, cru);
+    if (!kern) return COR_E_UNSUPPORTED;
+    int wps_max = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&wps_max, kern, 256, 0));
+    wps_max = std::max(1, std::min(wps_max, 8));
+    double t_best = 0.;
+    bool have = false;
+    for (int wps = 1; wps <= wps_max; wps++) {
+        for (int bt : {256, 64}) {
+            for (int qo : {1, 0}) {
+                cor_trace_opts o = *base;
+                o.step_limit = std::min<uint64_t>(base->step_limit, 100000ull);
+                o.waves_per_simd = wps;
+                o.block_threads = bt;
+                o.queue_order = qo;
+                cor_trace_plan *p = nullptr;
+                rc = cor_trace_plan_create(models, epoch, sample.data(), ns, &o, &p);
+                if (rc) return rc;
+                cor_trace_stats st;
+                rc = cor_trace_plan_run(p, nullptr);
+                if (!rc) rc = cor_trace_plan_stats(p, &st);
+                cor_trace_plan_destroy(p);
+                if (rc) return rc;
+                if (!have || st.kernel_ms < t_best) {
+                    have = true;
+                    t_best = st.kernel_ms;
+                    *best = *base;
+                    best->waves_per_simd = wps;
+                    best->block_threads = bt;
+                    best->queue_order = qo;
+                }
+            }
+        }
+    }
+    if (best_ms) *best_ms = t_best;
+    return COR_OK;
+}
 
 int64_t cor_trace_plan_count(const cor_trace_plan *p) { return p ? p->n_items : COR_E_INVALID; }
 
-int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
-    if (!p) return COR_E_INVALID;
-    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+}  // extern "C"
+
+namespace {
+// The launches of one run on stream s: queue and counter reset, the tracer, the
+// penumbra reduction; with timing, the plan's events around them.  No memset nodes, so
+// the sequence can be captured into a graph and replayed.
+hipError_t enqueue_run(cor_trace_plan *p, hipStream_t s, bool timing) {
     TraceArgs a;
     memset(&a, 0, sizeof a);
     a.epoch = p->d_epoch;
@@ -596,7 +758,6 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
     a.inv_mass = 1.0 / p->mass;
     a.out = p->out;
     a.coop_on = p->coop_on;
-    a.run_id = ++p->run_id;      // park records of earlier runs never match
     a.coop = p->d_coop;
     a.park = p->d_park;
     a.stats = p->d_run;
@@ -612,19 +773,45 @@ int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
         const double ticks = secs * (g_wall_khz > 0. ? g_wall_khz * 1e3 : 1e8);
         a.idle_guard = ticks < 1e18 ? (unsigned long long)ticks : 1000000000000000000ull;
     }
+    hipError_t e;
     reset_head_kernel<<<1, 1, 0, s>>>(p->d_head, p->d_coop, p->d_run);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(p->ev[0], s));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (timing && (e = hipEventRecord(p->ev[0], s)) != hipSuccess) return e;
     if (p->n_items > 0) {
         void *args[] = {&a};
-        HIP_TRY(hipLaunchKernel(p->kernel, dim3(p->grid), dim3(p->block), args, 0, s));
+        if ((e = hipLaunchKernel(p->kernel, dim3(p->grid), dim3(p->block), args, 0, s)) != hipSuccess) return e;
     }
-    HIP_TRY(hipEventRecord(p->ev[1], s));
+    if (timing && (e = hipEventRecord(p->ev[1], s)) != hipSuccess) return e;
     penumbra_kernel<<<p->n_scans, 64, 0, s>>>(p->d_scans, p->out.status, p->out.steps, p->out.attempts,
                                               p->d_cut, p->d_stat);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(p->ev[2], s));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (timing && (e = hipEventRecord(p->ev[2], s)) != hipSuccess) return e;
+    return hipSuccess;
+}
+}  // namespace
+
+extern "C" {
+
+int cor_trace_plan_run(cor_trace_plan *p, void *stream) {
+    if (!p) return COR_E_INVALID;
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    HIP_TRY(enqueue_run(p, s, true));
     p->ran = true;
+    return COR_OK;
+}
+
+int cor_trace_plan_progress(cor_trace_plan *p, int64_t *finished, int32_t *done) {
+    if (!p || !finished || !done || !p->ran) return COR_E_INVALID;
+    const hipError_t q = hipEventQuery(p->ev[2]);
+    if (q != hipSuccess && q != hipErrorNotReady) return fail_hip(q, "hipEventQuery");
+    *done = q == hipSuccess;
+    // a snapshot of the counter the running kernel advances, on a stream of its own
+    if (!p->s_progress) HIP_TRY(hipStreamCreateWithFlags(&p->s_progress, hipStreamNonBlocking));
+    if (!p->h_progress) HIP_TRY(hipHostMalloc((void **)&p->h_progress, sizeof(unsigned long long), hipHostMallocDefault));
+    HIP_TRY(hipMemcpyAsync(p->h_progress, &p->d_run->finished, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           p->s_progress));
+    HIP_TRY(hipStreamSynchronize(p->s_progress));
+    *finished = (int64_t)*p->h_progress;
     return COR_OK;
 }
 
@@ -715,7 +902,7 @@ int cor_trace_plan_stats(cor_trace_plan *p, cor_trace_stats *st) {
 
 int cor_trace_plan_set_epoch(cor_trace_plan *p, const cor_model_set *models, const ModelsParams *epoch,
                              void *stream) {
-    if (!p || !epoch) return COR_E_INVALID;
+    if (!p || !epoch || p->n_epochs != 1) return COR_E_INVALID;
     int intk, ext, cru, rc;
     if ((rc = model_kinds(models, &intk, &ext, &cru))) return rc;
     if (intk != p->intk || ext != p->ext || cru != p->cru) return COR_E_INVALID;
@@ -742,6 +929,108 @@ namespace {
 // streams of the epoch pipeline (slot 0 is the library stream); default priority
 constexpr int SERIES_MAX_IN_FLIGHT = 4;
 hipStream_t g_series_stream[SERIES_MAX_IN_FLIGHT] = {};
+
+// Offsets of the pinned result block of an epoch graph: the CoopCtl counters, the
+// per-scan cutoffs, then status, steps, attempts, handover and the eight doubles per
+// trajectory (the device's one-array-after-another layout).
+struct ResLayout {
+    size_t cut, status, steps, attempts, handover, dbl, total;
+};
+ResLayout res_layout(const cor_trace_plan *p) {
+    const size_t n = (size_t)p->n_items, nt = (size_t)std::max<int64_t>(p->n_items, 1);
+    ResLayout L;
+    L.cut = sizeof(CoopCtl);
+    L.status = L.cut + sizeof(cor_cutoff) * (size_t)p->n_scans;
+    L.steps = L.status + sizeof(int32_t) * n;
+    L.attempts = L.steps + sizeof(uint32_t) * n;
+    L.handover = L.attempts + sizeof(uint32_t) * n;
+    L.dbl = (L.handover + sizeof(uint32_t) * n + 7) & ~(size_t)7;
+    L.total = L.dbl + sizeof(double) * 8 * nt;
+    return L;
+}
+
+// Capture one epoch of the plan as a graph on stream s: the upload of the pinned epoch
+// block, the lane tables, the run (reset, trace, penumbra; no memset nodes) and the
+// copies of every result into the pinned result block.  Replayed once per epoch after
+// the host has rebuilt the epoch block.
+hipError_t capture_epoch_graph(cor_trace_plan *p, hipStream_t s) {
+    hipError_t e;
+    const ResLayout L = res_layout(p);
+    if (!p->h_epoch) {
+        if ((e = hipHostMalloc((void **)&p->h_epoch, sizeof(EpochDev), hipHostMallocDefault)) != hipSuccess) return e;
+    }
+    if ((e = hipHostMalloc((void **)&p->h_res, L.total, hipHostMallocDefault)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&p->ev_graph, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
+    const size_t n = (size_t)p->n_items, nt = (size_t)std::max<int64_t>(p->n_items, 1);
+    e = hipMemcpyAsync(p->d_epoch, p->h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && (p->coop_on || needs_coop_tables(p->prec, p->ext))) e = build_coop_tables(p->d_epoch, s);
+    if (e == hipSuccess) e = enqueue_run(p, s, false);
+    if (e == hipSuccess && p->d_coop)
+        e = hipMemcpyAsync(p->h_res, p->d_coop, sizeof(CoopCtl), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->h_res + L.cut, p->d_cut, sizeof(cor_cutoff) * p->n_scans, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && n > 0) {
+        e = hipMemcpyAsync(p->h_res + L.status, p->out.status, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(p->h_res + L.steps, p->out.steps, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(p->h_res + L.attempts, p->out.attempts, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(p->h_res + L.handover, p->out.handover, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(p->h_res + L.dbl, p->out.vc, sizeof(double) * 8 * nt, hipMemcpyDeviceToHost, s);
+    }
+    // always leave capture mode, also after a failed enqueue
+    hipGraph_t g = nullptr;
+    const hipError_t e_end = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return e;
+    }
+    if (e_end != hipSuccess) return e_end;
+    p->graph = g;
+    return hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+}
+
+// Results of the plan's last graph replay, from the pinned block (waits for it).
+int collect_graph(cor_trace_plan *p, cor_traj *traj, cor_cutoff *cut) {
+    HIP_TRY(hipEventSynchronize(p->ev_graph));
+    const ResLayout L = res_layout(p);
+    if (p->coop_on) {
+        const CoopCtl *c = (const CoopCtl *)p->h_res;
+        if (c->done != (unsigned long long)p->n_items) {
+            snprintf(g_hip_error, sizeof g_hip_error, "cooperative tail finished %llu of %lld trajectories",
+                     c->done, (long long)p->n_items);
+            return COR_E_HIP;
+        }
+    }
+    if (cut) memcpy(cut, p->h_res + L.cut, sizeof(cor_cutoff) * p->n_scans);
+    if (traj) {
+        const size_t n = (size_t)p->n_items, nt = (size_t)std::max<int64_t>(p->n_items, 1);
+        const int32_t *st = (const int32_t *)(p->h_res + L.status);
+        const uint32_t *sp = (const uint32_t *)(p->h_res + L.steps);
+        const uint32_t *at = (const uint32_t *)(p->h_res + L.attempts);
+        const uint32_t *ho = (const uint32_t *)(p->h_res + L.handover);
+        const double *blk = (const double *)(p->h_res + L.dbl);
+        for (size_t i = 0; i < n; i++) {
+            cor_traj &t = traj[i];
+            t.status = st[i];
+            t.steps = sp[i];
+            t.attempts = at[i];
+            t.handover = ho[i];
+            t.vc = blk[i];
+            t.r = blk[nt + i];
+            t.td = blk[2 * nt + i];
+            t.fei = blk[3 * nt + i];
+            t.ast = blk[4 * nt + i];
+            t.asf = blk[5 * nt + i];
+            t.time = blk[6 * nt + i];
+            t.length_m = blk[7 * nt + i];
+        }
+    }
+    return COR_OK;
+}
 }  // namespace
 
 int cor_gpu_trace_series(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
@@ -757,30 +1046,37 @@ int cor_gpu_trace_series(const cor_model_set *models, const ModelsParams *epochs
         if (!g_series_stream[k]) HIP_TRY(hipStreamCreateWithFlags(&g_series_stream[k], hipStreamNonBlocking));
     cor_trace_plan *plan[SERIES_MAX_IN_FLIGHT] = {};
     int64_t count = 0;
-    // epoch e runs in slot e % k_max; before a slot takes its next epoch, the results
-    // of the previous one are copied out, so up to k_max epochs are on the device and
-    // the tail of one overlaps the bulk of the next
+    // Each slot holds one plan and its epoch graph; epoch e replays slot e % k_max's
+    // graph after the host has collected that slot's previous epoch and rebuilt the
+    // pinned epoch block, so up to k_max epochs are on the device at once and the tail
+    // of one overlaps the bulk of the next.
     auto collect = [&](int slot, int e) -> int {
         cor_traj *t = traj ? traj + (size_t)e * (size_t)count : nullptr;
         cor_cutoff *c = cut ? cut + (size_t)e * (size_t)n_scans : nullptr;
-        return cor_trace_plan_results(plan[slot], t, c);
+        return collect_graph(plan[slot], t, c);
     };
     for (int e = 0; e < n_epochs && rc == COR_OK; e++) {
         const int slot = e % k_max;
+        hipStream_t s = g_series_stream[slot];
         if (!plan[slot]) {
             rc = cor_trace_plan_create(models, &epochs[e], scans, n_scans, opts, &plan[slot]);
-            if (rc == COR_OK) count = plan[slot]->n_items;
+            if (rc != COR_OK) break;
+            count = plan[slot]->n_items;
+            const hipError_t he = capture_epoch_graph(plan[slot], s);
+            if (he != hipSuccess) rc = fail_hip(he, "capture_epoch_graph");
         } else {
             rc = collect(slot, e - k_max);
-            if (rc == COR_OK) rc = cor_trace_plan_set_epoch(plan[slot], models, &epochs[e], g_series_stream[slot]);
         }
-        if (rc == COR_OK) rc = cor_trace_plan_run(plan[slot], g_series_stream[slot]);
+        if (rc != COR_OK) break;
+        cor_trace_plan *p = plan[slot];
+        build_epoch(models, &epochs[e], *p->h_epoch);
+        point_sh(models, p->d_sh, *p->h_epoch);
+        hipError_t he = hipGraphLaunch(p->gexec, s);
+        if (he == hipSuccess) he = hipEventRecord(p->ev_graph, s);
+        if (he != hipSuccess) rc = fail_hip(he, "hipGraphLaunch");
     }
     for (int e = std::max(0, n_epochs - k_max); e < n_epochs && rc == COR_OK; e++) rc = collect(e % k_max, e);
-    for (int k = 0; k < k_max; k++) {
-        if (plan[k] && plan[k]->ran) (void)hipEventSynchronize(plan[k]->ev[2]);
-        cor_trace_plan_destroy(plan[k]);
-    }
+    for (int k = 0; k < k_max; k++) cor_trace_plan_destroy(plan[k]);
     return rc;
 }
 
@@ -979,10 +1275,11 @@ int cor_gpu_rk_step(const double *h, const double *d, const double *field, doubl
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
                   double *geo_nT, int64_t n, int32_t precision) {
     if (!epoch || n < 0 || (n > 0 && (!gsm_m || !geo_nT))) return COR_E_INVALID;
-    if (precision < COR_PREC_FP64 || precision > COR_PREC_FP64_GROUP) return COR_E_INVALID;
+    if (precision < COR_PREC_FP64 || precision > COR_PREC_MIXED_GROUP4) return COR_E_INVALID;
     const bool coop = precision >= COR_PREC_MIXED_WAVE;
-    const bool coop64 = precision >= COR_PREC_FP64_WAVE;
-    const int lanes = precision == COR_PREC_MIXED_GROUP || precision == COR_PREC_FP64_GROUP ? COOP_GROUP
+    const bool coop64 = precision == COR_PREC_FP64_WAVE || precision == COR_PREC_FP64_GROUP;
+    const int lanes = precision == COR_PREC_MIXED_GROUP || precision == COR_PREC_FP64_GROUP ? 8
+                      : precision == COR_PREC_MIXED_GROUP4                                   ? 4
                       : precision == COR_PREC_MIXED_GROUP16                                  ? 16
                       : precision == COR_PREC_MIXED_GROUP32                                  ? 32
                                                                                              : 64;

@@ -94,9 +94,15 @@ struct ParkDev {
     uint32_t ready;              // = run id once the record is complete
 };
 
-// Lanes per trajectory when a cooperative worker runs several trajectories (eight to a
-// wavefront); a lone trajectory gets all 64.
+// Lanes per trajectory when a cooperative worker runs several trajectories: groups of 8
+// (eight to a wavefront); a lone trajectory gets all 64.  The evaluator also runs with
+// groups of 4 (sixteen to a wavefront; cor_gpu_field's COR_PREC_MIXED_GROUP4), but the
+// tracer is slower with them: on the 5-degree grid 17.7 s against 16.1 s, the trapped
+// trajectories' steps taking longer while the workers are crowded.
 constexpr int COOP_GROUP = 8;
+constexpr int COOP_GROUP_MIXED = 8;
+template <int PREC>
+constexpr int coop_group() { return PREC == PREC_MIXED ? COOP_GROUP_MIXED : COOP_GROUP; }
 
 // Work-handover counters of one run (cleared by the reset kernel).
 // The counters each sit in a cache line of their own: workers poll them while others
@@ -109,6 +115,9 @@ struct CoopCtl {
     alignas(128) unsigned long long done;     // trajectories finished (both modes)
     alignas(128) int32_t converting;          // wavefronts running down their lanes to turn worker
     alignas(128) int32_t feeders;             // wavefronts taking new trajectories from the queue
+    alignas(128) uint32_t run;                // run id: advanced by every reset, never cleared, so
+                                              // park records of earlier runs never match (a value
+                                              // the device keeps, so a captured run replays right)
 };
 
 struct ScanDev {
@@ -117,6 +126,8 @@ struct ScanDev {
     double rig, del, rk, zn;
     int32_t nk1, n;
     int64_t first;               // global id of rigidity index 0
+    int32_t epoch;               // multi-epoch launches: index of the scan's epoch block
+    int32_t pad;
 };
 
 struct TraceOut {
@@ -138,6 +149,8 @@ struct RunStats {
     unsigned long long t_start;      // earliest wavefront start
     unsigned long long t_drain;      // the queue's last slot taken
     unsigned long long t_lane_end;   // last wavefront to leave the per-lane mode
+    unsigned long long finished;     // trajectories ended so far (both modes; read by the
+                                     // host while the kernel runs, as its progress)
 };
 
 struct TraceArgs {
@@ -159,7 +172,7 @@ struct TraceArgs {
     // run handover_steps accepted steps in the per-lane mode is parked and continued by
     // cooperative workers, a group of lanes per trajectory (coop_on = 0: per-lane only)
     int32_t coop_on;
-    uint32_t run_id;
+    uint32_t pad2;
     CoopCtl *coop;
     ParkDev *park;
     RunStats *stats;

@@ -28,6 +28,9 @@ const void *trace_kernel_fp64(int intk, int ext, int cru);
 const void *trace_kernel_mixed(int intk, int ext, int cru);
 // the FP64 tracer with the cooperative tail (IGRF / T96 model sets; nullptr otherwise)
 const void *trace_kernel_fp64_coop(int intk, int ext, int cru);
+// the multi-epoch tracer (kernels_epochs_*.hip; model sets without T96, else nullptr)
+const void *trace_kernel_epochs_fp64(int intk, int ext, int cru);
+const void *trace_kernel_epochs_mixed(int intk, int ext, int cru);
 const void *field_kernel_fp64(int intk, int ext, int cru);
 const void *field_kernel_mixed(int intk, int ext, int cru);
 // Mixed-precision field batch evaluated like the tracer's cooperative mode (one

@@ -81,7 +81,10 @@ const void *field_coop_table_lanes(int intk, int ext, int cru, int lanes) {
     case 64: return field_coop_table<PREC, 64>(intk, ext, cru);
     case 32: return field_coop_table<PREC, 32>(intk, ext, cru);
     case 16: return field_coop_table<PREC, 16>(intk, ext, cru);
-    default: return field_coop_table<PREC, COOP_GROUP>(intk, ext, cru);
+    case 4:
+        if constexpr (PREC == PREC_MIXED) return field_coop_table<PREC, 4>(intk, ext, cru);
+        return nullptr;
+    default: return field_coop_table<PREC, 8>(intk, ext, cru);
     }
 }
 
@@ -100,6 +103,22 @@ const void *trace_kernel_table(int intk, int ext, int cru) {
 #define COR_PICK(I, E, C) \
     if (intk == I && ext == E && cru == C) return (const void *)&trace_kernel<PREC, I, E, C>;
     COR_MODEL_SETS(COR_PICK)
+#undef COR_PICK
+    return nullptr;
+}
+
+// the multi-epoch tracer: per-lane mode, model sets without T96 (whose constants the
+// per-lane evaluator keeps in registers)
+#define COR_EPOCHS_MODEL_SETS(X)           \
+    X(INT_IGRF, EXT_TS05, CRU_NONE)        \
+    X(INT_NONE, EXT_TS05, CRU_NONE)        \
+    X(INT_IGRF, EXT_NONE, CRU_NONE)
+
+template <int PREC>
+const void *trace_kernel_epochs_table(int intk, int ext, int cru) {
+#define COR_PICK(I, E, C) \
+    if (intk == I && ext == E && cru == C) return (const void *)&trace_kernel<PREC, I, E, C, false, true>;
+    COR_EPOCHS_MODEL_SETS(COR_PICK)
 #undef COR_PICK
     return nullptr;
 }

@@ -93,6 +93,30 @@ __device__ __forceinline__ EpochView<FT> stage_epoch(const EpochDev *__restrict_
     return V;
 }
 
+// The view of one epoch block read straight from device memory, for the multi-epoch
+// tracer whose lanes may each be at another epoch: the IGRF tables and the TS05 and SH
+// constants through pointers into the block (vector loads that the caches serve), the
+// rotation rows loaded.  Its T96 constants are not set: that tracer has no T96 model set.
+template <class FT>
+__device__ __forceinline__ EpochView<FT> global_view(const EpochDev *__restrict__ E) {
+    EpochView<FT> V;
+    V.G = igrf_tab(E, 0, FT(0));
+    V.H = igrf_tab(E, 1, FT(0));
+    V.REC = igrf_tab(E, 2, FT(0));
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        V.A[k] = E->A[k];
+        V.Af[k] = E->Af[k];
+    }
+    V.ts05 = ts05_of(E, FT(0));
+    V.ts05d = &E->ts05d;
+    V.shG = sh_tab(E, 0, FT(0));
+    V.shH = sh_tab(E, 1, FT(0));
+    V.sh_nm = E->sh_nm;
+    V.icon = &E->coop.icon[0][0];
+    return V;
+}
+
 // ---------------------------------------------------------------- field aggregates
 
 __device__ __forceinline__ v3<double> rot_geo_to_gsm(const double *A, v3<double> g) {
@@ -567,7 +591,7 @@ __device__ __forceinline__ unsigned long long load_relaxed_uniform(unsigned long
 // Hand this wavefront's live trajectories to the cooperative workers (called with the
 // whole wavefront active, at the top of a step): lane 0 reserves one park slot per live
 // lane, each live lane writes its state and publishes the slot with the run id.
-__device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool active, int64_t gid,
+__device__ __forceinline__ void park_lanes(const TraceArgs &a, uint32_t run_id, int lane, bool active, int64_t gid,
                                            v3<double> pos, v3<double> vel, double v, double time,
                                            double alength, double zn, unsigned int step, uint32_t attempts,
                                            int nk1) {
@@ -593,12 +617,12 @@ __device__ __forceinline__ void park_lanes(const TraceArgs &a, int lane, bool ac
         P->attempts = attempts;
         P->nk1 = nk1;
         P->handover = 0u;
-        __hip_atomic_store(&P->ready, a.run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&P->ready, run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 // The field of a wavefront's few trajectories with FG lanes each (fewer, wider groups
-// than the bookkeeping groups of COOP_GROUP lanes: a wavefront step costs about the same
+// than the bookkeeping groups of coop_group<PREC>() lanes: a wavefront step costs about the same
 // whatever the number of trajectories, and wider groups finish it sooner).  Field group
 // j evaluates the j-th live trajectory (leads: the first lanes of the bookkeeping groups
 // that hold one); each bookkeeping group then reads its trajectory's field from that
@@ -622,10 +646,10 @@ __device__ __forceinline__ v3<FT> coop_field_spread(const EpochView<FT> &E, cons
 
 // Cooperative worker; returns true when the run is over, false when the wavefront should
 // go back to the per-lane mode (it is idle, no parked trajectory waits and the queue still
-// has new ones).  A wavefront runs up to eight parked trajectories, one per group of
-// COOP_GROUP lanes, while parked work is plentiful; with four or fewer the field is
-// evaluated with wider groups (a lone trajectory with all 64 lanes: its step then takes
-// about two thirds of the time).  Groups of a wavefront hold the same state in all
+// has new ones).  A wavefront runs up to 64 / GS parked trajectories (sixteen in mixed
+// precision, eight in FP64), one per group of GS = coop_group<PREC>() lanes, while parked
+// work is plentiful; with eight or fewer the field is evaluated with wider groups (a lone
+// trajectory with all 64 lanes: its step then takes about two thirds of the time).  Groups of a wavefront hold the same state in all
 // their lanes and compute the same bits.
 //   * An idle wavefront takes one ticket for the next park slot (tickets are served in
 //     slot order) and waits until the slot is reserved and published.
@@ -644,10 +668,10 @@ __device__ __forceinline__ v3<FT> coop_field_spread(const EpochView<FT> &E, cons
 // fire while a trajectory is still running -- an idle wavefront retires and the host
 // reports the shortfall.  Workers never wait for one another and parkers never wait.
 template <int PREC, int INT, int EXT>
-__device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<PREC>::T> &E,
+__device__ bool coop_worker(const TraceArgs &a, uint32_t run_id, const EpochView<typename FieldT<PREC>::T> &E,
                             const CoopTabT<typename FieldT<PREC>::T> &T, int lane) {
     typedef typename FieldT<PREC>::T FT;
-    constexpr int GS = COOP_GROUP;
+    constexpr int GS = coop_group<PREC>();
     const int gl = lane & (GS - 1);
     const int lead = lane & ~(GS - 1);            // first lane of this lane's group
     const bool is_lead = gl == 0;
@@ -717,7 +741,7 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<
             }
             if (holding && ticket < res) {
                 ParkDev *P = a.park + ticket;
-                if (__hip_atomic_load(&P->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == a.run_id) {
+                if (__hip_atomic_load(&P->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == run_id) {
                     pos = mk3(P->pos[0], P->pos[1], P->pos[2]);
                     vel = mk3(P->vel[0], P->vel[1], P->vel[2]);
                     v = P->v;
@@ -793,7 +817,7 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<
                             P->nk1 = nk1;
                             P->handover = handover;
                             if (evals) atomicAdd(&a.stats->coop_steps, evals);
-                            __hip_atomic_store(&P->ready, a.run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_store(&P->ready, run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                         }
                         has = false;
                     }
@@ -804,8 +828,9 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<
             }
         }
         // ---- 3. one step of each trajectory.  The field: a lone trajectory with all 64
-        // lanes, two with 32 lanes each, three or four with 16, more with their own
-        // group of COOP_GROUP lanes (groups without one evaluate at their last point).
+        // lanes, two with 32 lanes each, three or four with 16, five to eight with 8, more
+        // with their own group of GS lanes (groups without one evaluate at their last
+        // point).
         v3<FT> f;
         const int k_live = __popcll(has_leads);
         if (k_live == 1) {
@@ -817,6 +842,8 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<
             f = coop_field_spread<FT, INT, EXT, 32>(E, T, has_leads, pos, lane, lead);
         } else if (k_live <= 4) {
             f = coop_field_spread<FT, INT, EXT, 16>(E, T, has_leads, pos, lane, lead);
+        } else if (GS < 8 && k_live <= 8) {
+            f = coop_field_spread<FT, INT, EXT, (GS < 8 ? 8 : GS)>(E, T, has_leads, pos, lane, lead);
         } else {
             const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
             f = coop_field<FT, INT, EXT, GS>(E.G, E.H, E.REC, rot_rows(E), E.t96, T, pos, gsm, gl);
@@ -836,6 +863,7 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<
                     write_end(a, gid, status, step, attempts, handover);
                     atomicAdd(&a.stats->coop_steps, evals);
                     atomicAdd(&a.coop->done, 1ull);
+                    __hip_atomic_fetch_add(&a.stats->finished, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 has = false;
             }
@@ -853,18 +881,29 @@ __device__ bool coop_worker(const TraceArgs &a, const EpochView<typename FieldT<
 
 // COOP: the kernel carries the cooperative worker (mixed precision always; FP64 in a
 // kernel of its own, so the FP64 per-lane kernel keeps its registers free of it).
-template <int PREC, int INT, int EXT, int CRU, bool COOP = PREC == PREC_MIXED>
+// MULTI: the multi-epoch tracer -- a.epoch is an array of epoch blocks and each scan
+// names its block, so one launch drains trajectories of many epochs; each lane reads
+// its epoch's constants from device memory (per-lane mode only, no T96 model set).
+template <int PREC, int INT, int EXT, int CRU, bool COOP = PREC == PREC_MIXED, bool MULTI = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COR_WAVES_PER_EU)))
 trace_kernel(TraceArgs a) {
     typedef typename FieldT<PREC>::T FT;
-    __shared__ FT s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
-    const EpochView<FT> E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
+    static_assert(!MULTI || (!COOP && EXT != EXT_T96), "the multi-epoch tracer is per-lane, without T96");
+    EpochView<FT> E;
+    if constexpr (!MULTI) {
+        __shared__ FT s_g[IGRF_LDS_N], s_h[IGRF_LDS_N], s_rec[IGRF_LDS_N];
+        E = stage_epoch<FT>(a.epoch, s_g, s_h, s_rec);
+    }
+    const EpochDev *ep = a.epoch;   // this lane's epoch block (multi-epoch tracer)
     const int lane = __lane_id();
     const CoopTabT<FT> *ctab = nullptr;
     constexpr bool kCoop = COOP && EXT != EXT_TS05 && CRU == CRU_NONE;   // the cooperative evaluator's models
+    // this run's id (advanced by the reset kernel that precedes every launch)
+    uint32_t run_id = 0u;
     if constexpr (kCoop) {
         __shared__ CoopTabT<FT> s_coop;
         if (a.coop_on) {
+            run_id = __hip_atomic_load(&a.coop->run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             stage_coop(coop_tab(a.epoch, FT(0)), &s_coop);
             if (lane == 0) atomicAdd(&a.coop->running, 1);
         }
@@ -927,6 +966,7 @@ trace_kernel(TraceArgs a) {
                             scan = find_scan(a.scans, a.n_scans, gid);
                             const ScanDev &S = a.scans[scan];
                             const int i = (int)(gid - S.first);
+                            if constexpr (MULTI) ep = a.epoch + S.epoch;
                             // src/trajectory.c:13-43
                             zn = S.zn;
                             nk1 = S.nk1;
@@ -955,11 +995,11 @@ trace_kernel(TraceArgs a) {
                     // same in every run, whatever the scheduling.
                     const bool ho = active && step >= a.handover_steps;
                     if (__ballot(ho)) {
-                        park_lanes(a, lane, ho, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
+                        park_lanes(a, run_id, lane, ho, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
                         if (ho) active = false;
                     }
                     // ---- 3. a feeding wavefront stops taking new trajectories when parked ones
-                    // wait for workers beyond what converting wavefronts will take (eight each),
+                    // wait for workers beyond what converting wavefronts will take (one per group),
                     // as long as min_feeders others still feed; it runs its lanes down (each at
                     // most handover_steps steps) and turns worker
                     if (feeding && (iter & 15u) == 0) {
@@ -972,7 +1012,8 @@ trace_kernel(TraceArgs a) {
                                 // concurrent conversions cannot take the count below it)
                                 const int f = atomicSub(&a.coop->feeders, 1);
                                 c = atomicAdd(&a.coop->converting, 1);
-                                if (f <= a.min_feeders || backlog <= 8ll * c + a.convert_backlog) {
+                                if (f <= a.min_feeders ||
+                                    backlog <= (long long)(64 / coop_group<PREC>()) * c + a.convert_backlog) {
                                     atomicSub(&a.coop->converting, 1);
                                     atomicAdd(&a.coop->feeders, 1);
                                     c = -1;
@@ -1001,16 +1042,21 @@ trace_kernel(TraceArgs a) {
 
             bool ended_now = false;
             if (active) {
+                // the epoch of this lane: the workgroup's staged view, or (multi-epoch)
+                // the lane's own block read from device memory
+                EpochView<FT> EL;
+                if constexpr (MULTI) EL = global_view<FT>(ep);
+                const EpochView<FT> &EV = MULTI ? EL : E;
                 // ---- 4. field at the current point (src/trajectory.c:37-38,147-148); the
                 // GSM position is recomputed from the GEO state (same arithmetic as the
                 // reference's geo_to_gsm after an accepted step)
-                const v3<double> gsm = rot_geo_to_gsm(E.A, pos);
+                const v3<double> gsm = rot_geo_to_gsm(EV.A, pos);
                 if constexpr (PREC == PREC_FP64) {
-                    const v3<double> f = field_fp64<INT, EXT, CRU>(E, gsm);
+                    const v3<double> f = field_fp64<INT, EXT, CRU>(EV, gsm);
                     fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
                     b = 1.E-09 * sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
                 } else {
-                    const v3<float> f = field_mixed<INT, EXT, CRU>(E, pos, gsm);
+                    const v3<float> f = field_mixed<INT, EXT, CRU>(EV, pos, gsm);
                     fld = mk3<FT>((FT)f.x, (FT)f.y, (FT)f.z);
                     b = inv_b_mixed(f);   // the mixed step takes 1/b
                 }
@@ -1018,21 +1064,28 @@ trace_kernel(TraceArgs a) {
                 // ---- 5. attempts until one is accepted or the trajectory ends
                 int status;
                 if constexpr (PREC == PREC_FP64) {
-                    status = attempts_fp64(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
+                    status = attempts_fp64(a, EV, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
                 } else {
-                    status = attempts_mixed(a, E, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
+                    status = attempts_mixed(a, EV, fld, pos, vel, v, b, zn, nk0, nk1, step, attempts, time, alength, gid);
                 }
                 if (status >= 0) {
                     write_end(a, gid, status, step, attempts);
                     active = false;
                     ended_now = true;
                 } else if constexpr (PREC == PREC_FP64) {
-                    if (a.trk) write_track(a, gid, step, E.A, pos, v, time, alength);
+                    if (a.trk) write_track(a, gid, step, EV.A, pos, v, time, alength);
                 }
             }
-            // trajectories ended here, added to the run's count when the wavefront leaves
-            // the per-lane mode
-            ended_here += __popcll(__ballot(ended_now));
+            // trajectories ended here: the progress counter now, the run's count (which
+            // the cooperative workers' exit test reads) when the wavefront leaves the
+            // per-lane mode
+            const uint64_t ended_mask = __ballot(ended_now);
+            if (ended_mask) {
+                ended_here += __popcll(ended_mask);
+                if (lane == 0)
+                    __hip_atomic_fetch_add(&a.stats->finished, (unsigned long long)__popcll(ended_mask),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
         if (lane == 0) {
             atomicAdd(&a.stats->wave_iters, wave_iters);
@@ -1049,7 +1102,7 @@ trace_kernel(TraceArgs a) {
                     if (converting) atomicSub(&a.coop->converting, 1);
                     __hip_atomic_fetch_add(&a.coop->running, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                if (coop_worker<PREC, INT, EXT>(a, E, *ctab, lane)) break;
+                if (coop_worker<PREC, INT, EXT>(a, run_id, E, *ctab, lane)) break;
                 continue;
             }
         }

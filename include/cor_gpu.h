@@ -33,15 +33,15 @@ enum {
     COR_E_NOMEM = -5
 };
 
-/* COR_PREC_MIXED_WAVE / _GROUP / _GROUP16 / _GROUP32: cor_gpu_field only -- the
- * mixed-precision model evaluated as the tracer's cooperative mode evaluates it: one
- * wavefront (64 lanes) per point, one group of 8 lanes (eight trajectories to a
- * wavefront), or 16 or 32 lanes (four or two to a wavefront).  COR_PREC_FP64_WAVE /
- * _GROUP: the same evaluator in FP64 (the FP64 tracer's cooperative tail), 64 or 8
- * lanes per point. */
+/* COR_PREC_MIXED_WAVE / _GROUP4 / _GROUP / _GROUP16 / _GROUP32: cor_gpu_field only --
+ * the mixed-precision model evaluated as the tracer's cooperative mode evaluates it:
+ * one wavefront (64 lanes) per point, one group of 4 lanes (sixteen trajectories to a
+ * wavefront), of 8 lanes (eight), or 16 or 32 lanes (four or two to a wavefront).
+ * COR_PREC_FP64_WAVE / _GROUP: the same evaluator in FP64 (the FP64 tracer's
+ * cooperative tail), 64 or 8 lanes per point. */
 enum { COR_PREC_FP64 = 0, COR_PREC_MIXED = 1, COR_PREC_MIXED_WAVE = 2, COR_PREC_MIXED_GROUP = 3,
        COR_PREC_MIXED_GROUP16 = 4, COR_PREC_MIXED_GROUP32 = 5, COR_PREC_FP64_WAVE = 6,
-       COR_PREC_FP64_GROUP = 7 };
+       COR_PREC_FP64_GROUP = 7, COR_PREC_MIXED_GROUP4 = 8 };
 enum { COR_INTERNAL_NONE = 0, COR_INTERNAL_IGRF = 1 };
 enum { COR_EXTERNAL_NONE = 0, COR_EXTERNAL_T96 = 1, COR_EXTERNAL_TS05 = 2 };
 enum { COR_CRUSTAL_NONE = 0, COR_CRUSTAL_SH = 1 };
@@ -89,6 +89,12 @@ typedef struct {
                                * cooperative mode (a group of lanes each); 0 = the library
                                * default (65536).  A fixed step count: the results do not
                                * depend on scheduling */
+    int32_t waves_per_simd;   /* resident wavefronts per SIMD of the persistent grid: 0 = as many
+                               * as the kernel's registers allow (the default), otherwise at most
+                               * that many.  Scheduling only: the results do not change */
+    int32_t block_threads;    /* threads per workgroup of the persistent grid: 0 = the library's
+                               * choice (256, or 64 for launches with fewer trajectories than
+                               * lanes), else 64 or 256.  Scheduling only */
 } cor_trace_opts;
 
 /* Per trajectory.  The doubles are set only for allowed trajectories: exit v/c, exit
@@ -156,6 +162,10 @@ int cor_trace_plan_create(const cor_model_set *models, const ModelsParams *epoch
  * (a hipStream_t; NULL = the library's own stream).  Asynchronous. */
 int cor_trace_plan_run(cor_trace_plan *plan, void *stream);
 int cor_trace_plan_sync(cor_trace_plan *plan);
+/* Progress of the plan's last run without waiting for it: *finished = trajectories
+ * ended so far (a snapshot of a counter the running kernel advances), *done = 1 once
+ * the run (tracer and reduction) is complete.  No counterpart in the reference. */
+int cor_trace_plan_progress(cor_trace_plan *plan, int64_t *finished, int32_t *done);
 int64_t cor_trace_plan_count(const cor_trace_plan *plan);
 /* Copy results to the host (either pointer may be NULL).  traj has
  * cor_trace_plan_count() entries in scan order; cut has n_scans. */
@@ -179,12 +189,30 @@ int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const 
 /* The same scans traced at each of n_epochs epochs (the reference runs its whole
  * cutoff path once per epoch: src/main.c:38-86 per input file).  Up to in_flight
  * epochs (0 = 3, at most 4) run at once on separate streams, so the slow tail of one
- * epoch overlaps the next.  traj (nullable) receives n_epochs x count entries, epoch
- * major; cut receives n_epochs x n_scans. */
+ * epoch overlaps the next.  Each stream's epoch is a HIP graph captured once (epoch
+ * upload, lane tables, queue reset, tracer, penumbra reduction, result copies) and
+ * replayed for every later epoch of that stream.  traj (nullable) receives
+ * n_epochs x count entries, epoch major; cut receives n_epochs x n_scans. */
 int cor_gpu_trace_series(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
                          const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
                          int32_t in_flight, cor_traj *traj, cor_cutoff *cut);
 
+/* The same scans at n_epochs epochs in ONE launch: the persistent tracer drains a single
+ * queue of (epoch, scan, rigidity) trajectories, each lane reading its epoch's constants
+ * from device memory, so the slow trajectories of every epoch run side by side instead
+ * of one epoch's tail after another's (the reference runs its whole cutoff path once
+ * per epoch: src/main.c:38-86 per input file).  Per-lane mode; model sets without T96
+ * (IGRF, TS05, IGRF + TS05; COR_E_UNSUPPORTED otherwise).  Results are epoch-major:
+ * traj has n_epochs x count entries (count = the scans' cor_scan_count sum), cut has
+ * n_epochs x n_scans; each epoch's results equal a single-epoch trace of it bit for
+ * bit.  The plan form (cor_trace_plan_create_epochs) runs, times and reads back like
+ * any plan; cor_trace_plan_set_epoch does not apply to it. */
+int cor_trace_plan_create_epochs(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
+                                 const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts,
+                                 cor_trace_plan **plan);
+int cor_gpu_trace_epochs(const cor_model_set *models, const ModelsParams *epochs, int32_t n_epochs,
+                         const cor_scan *scans, int32_t n_scans, const cor_trace_opts *opts, cor_traj *traj,
+                         cor_cutoff *cut);
 /* cor_gpu_trace for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation).  No counterpart in
  * the reference.  Nothing is allocated: the epoch block, the scan descriptors and the
@@ -219,6 +247,20 @@ int cor_gpu_trace_tracks(const cor_model_set *models, const ModelsParams *epoch,
                          int32_t n_scans, const cor_trace_opts *opts, int64_t *counts, int64_t *total,
                          cor_track_point *points, int64_t capacity);
 
+/* Autotuner (no counterpart in the reference, which has no launch parameters): picks
+ * the persistent tracer's launch parameters for a workload by timing candidate launches
+ * on a seeded sample of its scans (sample_scans of them, at most min(step_limit, 1e5)
+ * steps per trajectory so that the throughput phase dominates the timing):
+ *   waves_per_simd   1 .. the kernel's register limit (resident wavefronts per SIMD);
+ *   block_threads    64 or 256 threads per workgroup (one or four wavefronts sharing
+ *                    a workgroup's LDS-staged epoch tables);
+ *   queue_order      rigidity order or longest-expected first.
+ * Only parameters that change no result are tuned (handover_steps, which moves the mixed
+ * path's switch to the cooperative arithmetic, is left as given).  *best receives *base
+ * with the fastest candidate's values; *best_ms (nullable) its tracer time on the sample. */
+int cor_trace_autotune(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
+                       int32_t n_scans, const cor_trace_opts *base, int32_t sample_scans, uint64_t seed,
+                       cor_trace_opts *best, double *best_ms);
 /* calculate_field over n points: GSM position [m] (n x 3) -> GEO field [nT] (n x 3). */
 int cor_gpu_field(const cor_model_set *models, const ModelsParams *epoch, const double *gsm_m,
                   double *geo_nT, int64_t n, int32_t precision);

@@ -10,6 +10,9 @@
 //
 // Usage: opcount <sites.txt> <nk1> <step_limit> <internal> <external> [parmod...]
 //   sites.txt: one "r0 lat lon" line per site (geocentric, vertical incidence)
+//   COR_OPCOUNT_SH (environment, optional): a file holding an SH model table -- int32
+//   nm, then nm*nm doubles g and nm*nm doubles h (row n, column m) -- traced as the
+//   crustal model of the set (lib/chaos/chaos.c:118-227)
 // Output: one line per site: "site n sum_steps sum_attempts sum_ops field_evals"
 // plus a total line; trajectories are rigidities 0.01..20 GV in 0.01 GV steps.
 #include <math.h>
@@ -126,6 +129,23 @@ int main(int argc, char **argv) {
     memset(&m, 0, sizeof m);
     m.internal = atoi(argv[4]);
     m.external = atoi(argv[5]);
+    std::vector<cdouble> sh_g, sh_h;
+    if (const char *sh = getenv("COR_OPCOUNT_SH")) {
+        FILE *fs = fopen(sh, "rb");
+        int32_t nm = 0;
+        if (!fs || fread(&nm, sizeof nm, 1, fs) != 1 || nm < 2 || nm > 200) return 2;
+        std::vector<double> g((size_t)nm * nm), h((size_t)nm * nm);
+        if (fread(g.data(), sizeof(double), g.size(), fs) != g.size() ||
+            fread(h.data(), sizeof(double), h.size(), fs) != h.size())
+            return 2;
+        fclose(fs);
+        sh_g.assign(g.begin(), g.end());
+        sh_h.assign(h.begin(), h.end());
+        m.crustal = O_CRU_CHAOS;
+        m.chaos_nm_max = nm;
+        m.chaos_g = sh_g.data();
+        m.chaos_h = sh_h.data();
+    }
     o_params p;
     memset((void *)&p, 0, sizeof p);
     o_igrf14_recalc(2020, 1, 1, 1, 0, 0, 0, &p);

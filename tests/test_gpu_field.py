@@ -80,10 +80,10 @@ def test_field_mixed_close_to_oracle(cor, oracle, name, kinds, parmod, pts):
     assert np.quantile(err, 0.99) < 1e-4
 
 
-@pytest.mark.parametrize("lanes", [64, 32, 16, 8])
+@pytest.mark.parametrize("lanes", [64, 32, 16, 8, 4])
 @pytest.mark.parametrize("name,kinds,parmod,pts", CASES, ids=[c[0] for c in CASES])
 def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts, lanes):
-    """The tracer's cooperative evaluator (one wavefront, or one group of 32, 16 or 8
+    """The tracer's cooperative evaluator (one wavefront, or one group of 32, 16, 8 or 4
     lanes, per point; term families in SIMD passes; one cross-lane sum): same FP32 bar
     as the per-lane mixed path.  With smaller groups, the points of a wavefront are
     different points, so the groups follow different model branches."""
@@ -93,7 +93,7 @@ def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts, lanes
         for k, v in enumerate(parmod):
             po.PARMOD[k] = v
     prec = {64: cor.PREC_MIXED_WAVE, 32: cor.PREC_MIXED_GROUP32, 16: cor.PREC_MIXED_GROUP16,
-            8: cor.PREC_MIXED_GROUP}[lanes]
+            8: cor.PREC_MIXED_GROUP, 4: cor.PREC_MIXED_GROUP4}[lanes]
     got = cor.field(cor.model_set(**kinds), p, pts, prec)
     want = oracle.field(L.models(**kinds), po, pts)
     r = np.linalg.norm(pts, axis=1) / RE
@@ -109,7 +109,7 @@ def test_field_wave_close_to_oracle(cor, oracle, name, kinds, parmod, pts, lanes
 def test_cooperative_field_same_bits_in_every_group_width(cor, name, kinds, parmod, pts):
     """The cooperative evaluator sums its terms in one canonical order (term t in slot
     brev6(t) of 64, slots summed pairwise in slot order) whatever the number of lanes
-    per point, so 64, 32, 16 and 8 lanes give bit-identical fields -- the property that
+    per point, so 64, 32, 16, 8 and 4 lanes give bit-identical fields -- the property that
     makes a trajectory's results independent of how many trajectories shared its
     wavefront.  The points include the magnetopause blend, the Region-1 band edges and
     the Region-2 xi bands (the box spans the T96 domain)."""
@@ -117,8 +117,8 @@ def test_cooperative_field_same_bits_in_every_group_width(cor, name, kinds, parm
     ms = cor.model_set(**kinds)
     outs = {lanes: cor.field(ms, p, pts, prec) for lanes, prec in
             ((64, cor.PREC_MIXED_WAVE), (32, cor.PREC_MIXED_GROUP32), (16, cor.PREC_MIXED_GROUP16),
-             (8, cor.PREC_MIXED_GROUP))}
-    for lanes in (32, 16, 8):
+             (8, cor.PREC_MIXED_GROUP), (4, cor.PREC_MIXED_GROUP4))}
+    for lanes in (32, 16, 8, 4):
         same = outs[lanes] == outs[64]
         print(name, lanes, "lanes: identical", same.all(axis=1).mean())
         assert same.all(), np.argwhere(~same.all(axis=1))[:5].ravel()

@@ -66,6 +66,66 @@ def test_ts05_series_matches_single_epochs(cor, in_flight):
     assert len({tuple(np.round(c["rc"], 6)) for c in cut}) > 1
 
 
+@pytest.mark.parametrize("precision", [1, 0], ids=["mixed-cooperative", "fp64"])
+def test_t96_series_graph_replays_match_eager_traces(cor, precision):
+    """IGRF + T96 at five epochs with two in flight: each in-flight slot captures its
+    epoch once as a graph (epoch upload, lane tables, reset, trace, penumbra, result
+    copies) and replays it for the later epochs, with the host collecting results and
+    rebuilding the epoch block between replays.  Every replay equals an eager trace of
+    its epoch bit for bit -- in the mixed path with the cooperative tail on (every
+    trajectory handed over after 20 steps), whose park records carry a run id that the
+    device advances at every replay."""
+    drivers = [[2, -20, 0, -2], [8, -250, 4, -12], [4, -80, -3, 5], [2, -20, 0, -2], [6, -150, 1, -8]]
+    params = [cor.recalc(2020, 1, 1, 1, 3 * k, 10 * k, 0, parmod=pm) for k, pm in enumerate(drivers)]
+    rows = site_rows(cor, SITES, rig=0.5, dl=0.25, nk1=100)
+    ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
+    kw = dict(step_limit=100000 if precision else 20000, precision=precision, handover_steps=20 if precision else 0)
+    tr, cut = cor.trace_series(ms, params, rows, in_flight=2, **kw)
+    for e, p in enumerate(params):
+        t1, c1 = cor.trace(ms, p, rows, **kw)
+        assert np.array_equal(tr[e], t1), e
+        assert np.array_equal(cut[e], c1), e
+    if precision:
+        assert (tr["handover"] > 0).any()
+    assert len({tuple(np.round(c["rc"], 6)) for c in cut}) > 1
+
+
+@pytest.mark.parametrize("precision,ext", [(0, 2), (1, 2), (1, 0)], ids=["fp64-ts05", "mixed-ts05", "mixed-igrf"])
+def test_multi_epoch_launch_equals_single_epochs(cor, precision, ext):
+    """Six storm epochs x three sites in ONE launch (cor_gpu_trace_epochs: a single queue
+    of (epoch, scan, rigidity) trajectories, each lane reading its epoch's block): every
+    epoch equals a single-epoch trace of it bit for bit, in FP64 and in mixed precision,
+    with both queue orders; the plan form gives the same."""
+    ep = storm_epochs(cor, [0, 95, 185, 750, 1200, 1435])
+    params = [cor.recalc(*t, parmod=pm) for t, pm in ep]
+    rows = site_rows(cor, SITES)
+    ms = cor.model_set(cor.INTERNAL_IGRF, ext)
+    for qo in (0, 1):
+        tr, cut = cor.trace_epochs(ms, params, rows, step_limit=20000, precision=precision, queue_order=qo)
+        assert tr.shape == (6, 3 * 39) and cut.shape == (6, 3)
+        for e, p in enumerate(params):
+            t1, c1 = cor.trace(ms, p, rows, step_limit=20000, precision=precision, coop=-1)
+            assert np.array_equal(tr[e], t1), (qo, e)
+            assert np.array_equal(cut[e], c1), (qo, e)
+    plan = cor.Plan(ms, params, rows, step_limit=20000, precision=precision, queue_order=1)
+    plan.run()
+    tp, cp = plan.results()
+    st = plan.stats()
+    plan.close()
+    assert np.array_equal(tp, tr) and np.array_equal(cp, cut)
+    assert st.n_traj == 6 * 3 * 39 and st.steps == int(tr["steps"].sum())
+    if ext == 2:
+        assert len({tuple(np.round(c["rc"], 6)) for c in cut}) > 1
+
+
+def test_multi_epoch_launch_rejects_t96(cor):
+    """The multi-epoch tracer has no T96 model set (its per-lane evaluator keeps the
+    epoch's T96 constants in registers): the call reports it instead of tracing."""
+    params = [cor.recalc(2020, 1, 1, 1, h, 0, 0, parmod=[2, -20, 0, -2]) for h in (0, 1)]
+    with pytest.raises(cor.CorError, match="cor_gpu_trace_epochs"):
+        cor.trace_epochs(cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96), params, site_rows(cor, SITES[:1]))
+
+
 def test_ts05_series_against_oracle(cor, oracle):
     """Two storm epochs at one site against the oracle, FP64 bar."""
     ep = storm_epochs(cor, [0, 750])

@@ -330,3 +330,23 @@ def test_grid_sites_at_benchmark_resolution_against_reference(cor, precision, co
         within += drc <= 0.01 + 1e-9
         assert drc <= max(0.01, 0.5 * (rcut[1] - rcut[0])) + 1e-9, name
     assert within >= 4
+
+
+def test_autotuner_picks_result_neutral_launch_parameters(cor):
+    """cor_trace_autotune on a seeded sample of 24 grid scans (IGRF + T96, mixed): the
+    chosen waves per SIMD, block size and queue order are valid, and a plan launched
+    with them gives the default launch's results bit for bit (none of the tuned
+    parameters changes a trajectory's arithmetic)."""
+    sites = cor.grid_sites(15.0, 30.0, 20.0)
+    rows = cor.scans_array([(r0, la, lo, la, lo, 0.2, 0.2, 20.0, -1.0, 100) for (r0, la, lo) in sites])
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
+    ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
+    tuned = cor.autotune(ms, p, rows, precision=cor.PREC_MIXED, step_limit=50000, sample_scans=24, seed=7)
+    print("autotuned", tuned)
+    assert tuned["waves_per_simd"] >= 1 and tuned["block_threads"] in (64, 256) and tuned["queue_order"] in (0, 1)
+    assert tuned["sample_ms"] > 0
+    kw = dict(step_limit=50000, precision=cor.PREC_MIXED)
+    t0, c0 = cor.trace(ms, p, rows, **kw)
+    t1, c1 = cor.trace(ms, p, rows, waves_per_simd=tuned["waves_per_simd"], block_threads=tuned["block_threads"],
+                       queue_order=tuned["queue_order"], **kw)
+    assert np.array_equal(t0, t1) and np.array_equal(c0, c1)

@@ -232,19 +232,19 @@ def test_gpu_calls_fail_cleanly_without_a_gpu():
 
 
 def test_field_precision_checked_first():
-    """cor_gpu_field rejects a precision outside FP64, mixed, the four mixed cooperative
-    evaluators (64, 8, 16, 32 lanes) and the two FP64 ones (64, 8) before any device
+    """cor_gpu_field rejects a precision outside FP64, mixed, the five mixed cooperative
+    evaluators (64, 8, 16, 32, 4 lanes) and the two FP64 ones (64, 8) before any device
     work."""
     lib = A.lib()
     p = A.recalc(2020)
     out = np.zeros(3)
     pts = np.zeros(3)
-    for bad in (-1, 8, 99):
+    for bad in (-1, 9, 99):
         rc = lib.cor_gpu_field(C.byref(A.model_set()), C.byref(p), pts.ctypes.data_as(A.DP),
                                out.ctypes.data_as(A.DP), 1, bad)
         assert rc == A.COR_E_INVALID, bad
     assert (A.PREC_MIXED_GROUP, A.PREC_MIXED_GROUP16, A.PREC_MIXED_GROUP32) == (3, 4, 5)
-    assert (A.PREC_FP64_WAVE, A.PREC_FP64_GROUP) == (6, 7)
+    assert (A.PREC_FP64_WAVE, A.PREC_FP64_GROUP, A.PREC_MIXED_GROUP4) == (6, 7, 8)
 
 
 def test_epoch_series_argument_checks():
