@@ -257,6 +257,12 @@ def main():
     ap.add_argument("--handover-steps", type=int, default=0,
                     help="accepted per-lane steps before a trajectory continues in the cooperative "
                          "mode (0: library default)")
+    ap.add_argument("--handover-low-steps", type=int, default=0,
+                    help="per-lane steps before the handover for the lowest rigidities of each scan "
+                         "(0: library default)")
+    ap.add_argument("--handover-low-count", type=int, default=0,
+                    help="how many of each scan's lowest rigidities --handover-low-steps covers (0: library "
+                         "default, negative: none)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sites", type=int, default=20, help="seeded grid sites the reference traces")
     ap.add_argument("--no-cpu", action="store_true")
@@ -285,7 +291,8 @@ def main():
     params = A.recalc(*EPOCH, parmod=PARMOD)
     models = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
     mixed = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_MIXED,
-                   queue_order=args.queue_order, coop=args.coop, handover_steps=args.handover_steps)
+                   queue_order=args.queue_order, coop=args.coop, handover_steps=args.handover_steps,
+                   handover_low_steps=args.handover_low_steps, handover_low_count=args.handover_low_count)
     n = mixed.n
     log(f"{len(scans)} sites, {n} trajectories; warmup {args.warmup}, steps {args.steps}")
     t_mixed, k_mixed = time_plan(mixed, args.steps, args.warmup)
@@ -308,7 +315,8 @@ def main():
         # cooperative evaluator sums the same terms in another order than the per-lane
         # FP64 path, which keeps the reference's; both are FP64 throughout)
         p64 = A.Plan(models, params, scans, step_limit=args.step_limit, precision=A.PREC_FP64,
-                     queue_order=args.queue_order, coop=1, handover_steps=args.handover_steps)
+                     queue_order=args.queue_order, coop=1, handover_steps=args.handover_steps,
+                     handover_low_steps=args.handover_low_steps, handover_low_count=args.handover_low_count)
         log(f"fp64 leg: the whole grid, {n} trajectories")
         t64, k64 = time_plan(p64, 1, 0)
         st_d = p64.stats()
@@ -385,7 +393,8 @@ def main():
                    "sites": len(scans), "site_stride": args.stride, "trajectories": int(n),
                    "nk1": args.nk1, "step_limit": args.step_limit, "tu_angle": 0.01,
                    "queue_order": args.queue_order, "coop": args.coop,
-                   "handover_steps": args.handover_steps},
+                   "handover_steps": args.handover_steps, "handover_low_steps": args.handover_low_steps,
+                   "handover_low_count": args.handover_low_count},
         "kernel_ms": k_mixed,
         "integration_steps_per_s": st_m.steps / (k_mixed * 1e-3),
         # where the tracer's time goes: mean live lanes per per-lane wavefront step,

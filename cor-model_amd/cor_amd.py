@@ -81,7 +81,8 @@ class Scan(C.Structure):
 class TraceOpts(C.Structure):
     _fields_ = [("step_limit", C.c_uint64), ("tu_angle", C.c_double), ("particle_mass", C.c_double),
                 ("precision", C.c_int32), ("queue_order", C.c_int32), ("coop", C.c_int32),
-                ("handover_steps", C.c_int32), ("waves_per_simd", C.c_int32), ("block_threads", C.c_int32)]
+                ("handover_steps", C.c_int32), ("waves_per_simd", C.c_int32), ("block_threads", C.c_int32),
+                ("handover_low_steps", C.c_int32), ("handover_low_count", C.c_int32)]
 
 
 class Cutoff(C.Structure):
@@ -238,10 +239,11 @@ class Plan:
     results() come back epoch-major as (n_epochs, count) and (n_epochs, n_scans)."""
 
     def __init__(self, models, params, scans, step_limit=1000000, tu_angle=0.01, mass=M_PROTON,
-                 precision=PREC_FP64, queue_order=0, coop=0, handover_steps=0, waves_per_simd=0, block_threads=0):
+                 precision=PREC_FP64, queue_order=0, coop=0, handover_steps=0, waves_per_simd=0, block_threads=0,
+                 handover_low_steps=0, handover_low_count=0):
         self.scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
         opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, coop, handover_steps, waves_per_simd,
-                         block_threads)
+                         block_threads, handover_low_steps, handover_low_count)
         h = C.c_void_p()
         self._models = models
         self.multi = isinstance(params, (list, tuple))
@@ -312,13 +314,15 @@ def trace(models, params, scans, **kw):
 
 
 def trace_series(models, epochs, scans, in_flight=0, step_limit=1000000, tu_angle=0.01, mass=M_PROTON,
-                 precision=PREC_FP64, queue_order=0, coop=0, handover_steps=0, traj=True):
+                 precision=PREC_FP64, queue_order=0, coop=0, handover_steps=0, traj=True, handover_low_steps=0,
+                 handover_low_count=0):
     """The scans traced at every epoch (a list of ModelsParams): per-trajectory results
     of shape (n_epochs, count) and cutoffs of shape (n_epochs, n_scans)."""
     scans = np.ascontiguousarray(scans, dtype=SCAN_DTYPE)
     count = sum(scan_count(r) for r in scans)
     arr = (ModelsParams * len(epochs))(*epochs)
-    opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, coop, handover_steps)
+    opts = TraceOpts(step_limit, tu_angle, mass, precision, queue_order, coop, handover_steps, 0, 0,
+                     handover_low_steps, handover_low_count)
     cut = np.zeros((len(epochs), len(scans)), dtype=CUT_DTYPE)
     tr = np.zeros((len(epochs), count), dtype=TRAJ_DTYPE) if traj else None
     check(lib().cor_gpu_trace_series(C.byref(models), C.cast(arr, C.c_void_p), len(epochs), scans.ctypes.data,

@@ -212,7 +212,7 @@ static int cutoff(const Cli *c, const Model **models) {
             }
         if (in[k].group < 0) in[k].group = n_groups++;
     }
-    const cor_trace_opts opts = {steps, tu, mass, c->precision, 1, 0, 0, 0, 0};
+    const cor_trace_opts opts = {steps, tu, mass, c->precision, 1, 0, 0, 0, 0, 0, 0};
     for (int g = 0; g < n_groups && !rc; g++) {
         int ns = 0;
         int64_t total = 0;

@@ -310,6 +310,14 @@ NoAllocHost g_noalloc_host[NOALLOC_MAX_DEVICES];
 // is taken because a single scan, whose run time is the latency of its longest
 // trajectory, then leaves the per-lane mode sooner.
 constexpr uint32_t COOP_HANDOVER_DEFAULT = 65536u;
+// An earlier handover for the lowest rigidities of each scan, which hold most of the
+// trajectories trapped until the step limit (on the 5-degree grid 1468 of 1852 lie among
+// the first 20 of 2000), is off by default: it moves more work into the cooperative mode
+// than it saves on the trapped trajectories' path.  Full grid, measured against 15.9 s
+// with every trajectory at 65536: 16.8 s (32768 for the lowest 20), 16.8 s (16384, 20),
+// 16.6 s (32768, 50), 16.8 s (98304 for the others, 32768 for the lowest 20).
+constexpr uint32_t COOP_HANDOVER_LOW_DEFAULT = 32768u;
+constexpr int32_t COOP_HANDOVER_LOW_COUNT_DEFAULT = 0;
 // Parked trajectories waiting for workers, beyond eight per converting wavefront, that
 // make a feeding wavefront turn worker; and the share of wavefronts that keep feeding
 // until the queue is drained (three quarters: 16.2 s on the full grid, against 17.9 s
@@ -345,6 +353,8 @@ struct cor_trace_plan {
     int grid, block;
     int coop_on;                 // 0: per-lane mode only
     uint32_t handover_steps;     // per-lane accepted steps before the cooperative mode
+    uint32_t handover_low_steps; // ... for the lowest handover_low_count rigidities of a scan
+    int32_t handover_low_count;
     size_t park_slots;
     CoopCtl *d_coop;
     ParkDev *d_park;
@@ -604,6 +614,13 @@ int plan_create_impl(const cor_model_set *models, const ModelsParams *epochs, in
     p->coop_on = !multi && (p->prec == COR_PREC_MIXED ? opts->coop >= 0 : opts->coop > 0) && ext != EXT_TS05 &&
                  cru == CRU_NONE;
     p->handover_steps = opts->handover_steps > 0 ? (uint32_t)opts->handover_steps : COOP_HANDOVER_DEFAULT;
+    // the lowest rigidities hand over no later than the others
+    p->handover_low_steps = opts->handover_low_steps > 0   ? (uint32_t)opts->handover_low_steps
+                            : opts->handover_low_steps < 0 ? p->handover_steps
+                                                           : std::min(COOP_HANDOVER_LOW_DEFAULT, p->handover_steps);
+    p->handover_low_count = opts->handover_low_count > 0   ? opts->handover_low_count
+                            : opts->handover_low_count < 0 ? 0
+                                                           : COOP_HANDOVER_LOW_COUNT_DEFAULT;
 
     // persistent grid: as many resident blocks as the occupancy allows, but no more
     // blocks than are needed to give every trajectory a lane at the start (or, with the
@@ -765,6 +782,8 @@ hipError_t enqueue_run(cor_trace_plan *p, hipStream_t s, bool timing) {
     a.trk = p->d_trk;
     a.trk_off = p->d_trk_off;
     a.handover_steps = p->handover_steps;
+    a.handover_low_steps = p->handover_low_steps;
+    a.handover_low_count = p->handover_low_count;
     a.convert_backlog = COOP_CONVERT_BACKLOG;
     a.min_feeders = std::max(1, (int)((int64_t)p->grid * (p->block / 64) * 3 / 4));
     {

@@ -180,6 +180,8 @@ struct TraceArgs {
     cor_track_point *trk;        // FP64 track recording (NULL: off)
     const int64_t *trk_off;      // first point of each trajectory (n_items + 1 entries)
     uint32_t handover_steps;     // per-lane accepted steps before the handover
+    uint32_t handover_low_steps; // ... for the first handover_low_count rigidities of a scan
+    int32_t handover_low_count;
     int32_t convert_backlog;     // parked trajectories waiting, beyond what converting
                                  // wavefronts will take, that turn a feeding wavefront
                                  // into a worker

@@ -922,6 +922,7 @@ trace_kernel(TraceArgs a) {
     int nk0 = 0, nk1 = 0;
     unsigned int step = 0;
     uint32_t attempts = 0;
+    uint32_t ho_at = 0xffffffffu;   // this trajectory's handover step (cooperative mode)
     unsigned iter = 0;
     // fill of this wavefront's per-lane loop
     unsigned long long wave_iters = 0, lane_iters = 0, wave_iters_open = 0, lane_iters_open = 0;
@@ -971,6 +972,7 @@ trace_kernel(TraceArgs a) {
                             zn = S.zn;
                             nk1 = S.nk1;
                             nk0 = nk1;
+                            ho_at = i < a.handover_low_count ? a.handover_low_steps : a.handover_steps;
                             const double rig = S.rig + i * S.del;
                             const double aa = rig * 1.0e+09 * zn * K_Q / (a.mass * K_C * K_C);
                             v = K_C * sqrt(1. - 1. / (1. + aa * aa));
@@ -988,12 +990,14 @@ trace_kernel(TraceArgs a) {
             }
             if constexpr (kCoop) {
                 if (a.coop_on) {
-                    // ---- 2. handover: a trajectory that has run handover_steps accepted steps
-                    // here continues in the cooperative mode.  The step count alone decides,
-                    // so each trajectory's arithmetic (per-lane up to that step, cooperative
-                    // after it, whose sums have one order for every group width) is the
-                    // same in every run, whatever the scheduling.
-                    const bool ho = active && step >= a.handover_steps;
+                    // ---- 2. handover: a trajectory that has run its handover step count
+                    // (handover_low_steps for the lowest rigidities of a scan, where trapped
+                    // trajectories concentrate, handover_steps for the others) continues in
+                    // the cooperative mode.  The trajectory's own step count and rigidity
+                    // index alone decide, so its arithmetic (per-lane up to that step,
+                    // cooperative after it, whose sums have one order for every group width)
+                    // is the same in every run, whatever the scheduling.
+                    const bool ho = active && step >= ho_at;
                     if (__ballot(ho)) {
                         park_lanes(a, run_id, lane, ho, gid, pos, vel, v, time, alength, zn, step, attempts, nk1);
                         if (ho) active = false;

@@ -26,7 +26,7 @@ void trajectory_simulation_cutoff(InfileHeader *header, const Model **models, Mo
     cor_model_set set;
     const cor_scan scan = {header->r0, header->the0, header->fi0, header->the1, header->fi1,
                            header->rig, header->del, header->rk, header->zn, header->nk1, 0};
-    const cor_trace_opts opts = {step_limit, tu_angle, particle_mass, COR_PREC_FP64, 0, 0, 0, 0, 0};
+    const cor_trace_opts opts = {step_limit, tu_angle, particle_mass, COR_PREC_FP64, 0, 0, 0, 0, 0, 0, 0};
     int rc = cor_models_to_set(models, &set);
     const int64_t n = cor_scan_count(&scan);
     if (rc == COR_OK && n < 0) rc = COR_E_INVALID;
@@ -55,7 +55,7 @@ void trajectory_simulation(InfileHeader *header, const Model **models, ModelsPar
     cor_model_set set;
     const cor_scan scan = {header->r0, header->the0, header->fi0, header->the1, header->fi1,
                            header->rig, header->del, header->rk, header->zn, header->nk1, 0};
-    const cor_trace_opts opts = {step_limit, tu_angle, particle_mass, COR_PREC_FP64, 0, 0, 0, 0, 0};
+    const cor_trace_opts opts = {step_limit, tu_angle, particle_mass, COR_PREC_FP64, 0, 0, 0, 0, 0, 0, 0};
     int rc = cor_models_to_set(models, &set);
     const int64_t n = cor_scan_count(&scan);
     if (rc == COR_OK && n < 0) rc = COR_E_INVALID;

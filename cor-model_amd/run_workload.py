@@ -7,14 +7,15 @@ stderr).  All inputs are synthetic and seeded; the field models are the library'
               [-300, 0] nT; By, Bz ~ N(0, 5) nT; W1..W6 U(0, 2); the dipole tilt from
               each epoch's time), IGRF-14 + TS05, 50 seeded sites (area-uniform) at
               20 km, vertical, 0.01-20 GV in 0.01 GV steps: 28.8 M trajectories in one
-              multi-epoch launch, mixed precision and FP64; dRc per site per epoch.
+              multi-epoch launch in mixed precision; mixed and FP64 on the first
+              epochs (at a reduced step limit), dRc per site per epoch.
   map         high-resolution global map: a degree-20 spherical-harmonic core field
               (IGRF-14 2020.0 Gauss coefficients with a seeded +-1 % perturbation for
               degrees 1..13, seeded coefficients continuing the spectrum's decay for
               14..20) as the SH model, plus TS05 (Pdyn 3, Dst -100, By 2, Bz -5,
               W1..W6 0.5); 1-degree grid (181 x 360 = 65 160 sites) at 20 km, vertical,
               0.02-20 GV in 0.02 GV steps (65.2 M trajectories), mixed precision with
-              autotuned launch parameters, FP64 rerun on a 1 % seeded subsample.
+              autotuned launch parameters; mixed and FP64 on a 1 % seeded subsample.
   directions  apparent cutoffs and asymptotic directions: IGRF-14 + T96 (Pdyn 2,
               Dst -20, By 0, Bz -2), 64 seeded sites (latitude U(-60, 60), longitude
               U(0, 360)) at 20 km, 193 directions each (vertical plus zenith 10..80 x
@@ -153,30 +154,33 @@ def run_storm(args):
         m = 5 * k
         params.append(A.recalc(2020, 1, 1, 1, m // 60, m % 60, 0, parmod=pm))
     ms = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_TS05)
-    kw = dict(step_limit=args.step_limit, queue_order=1)
     log(f"storm: {args.epochs} epochs x {args.sites} sites x 2000 rigidities")
-    pm = A.Plan(ms, params, rows, precision=A.PREC_MIXED, **kw)
-    st_m, wall_m = timed_run(pm, "mixed")
-    tm, cm = pm.results()
+    pm = A.Plan(ms, params, rows, precision=A.PREC_MIXED, step_limit=args.step_limit, queue_order=1)
+    st_m, wall_m = timed_run(pm, "mixed, all epochs")
+    _, cm = pm.results(traj=False)
     pm.close()
     out = {"workload": "TS05 storm series (IGRF-14 + TS05), one multi-epoch launch",
            "epochs": args.epochs, "sites": args.sites, "nk1": args.nk1, "step_limit": args.step_limit,
-           "mixed": rate_block(st_m, "ts05_igrf"), "mixed_wall_s": wall_m}
+           "mixed": rate_block(st_m, "ts05_igrf"), "mixed_wall_s": wall_m,
+           "rc_range_GV": [float(cm["rc"].min()), float(cm["rc"].max())]}
     if not args.no_fp64:
-        ne = args.fp64_epochs or args.epochs
-        pd = A.Plan(ms, params[:ne], rows, precision=A.PREC_FP64, **kw)
-        st_d, wall_d = timed_run(pd, f"fp64 ({ne} epochs)")
-        td, cd = pd.results()
-        pd.close()
-        out["fp64"] = rate_block(st_d, "ts05_igrf")
-        out["fp64"]["epochs"] = ne
-        out["speedup_mixed_vs_fp64"] = (out["mixed"]["trajectories_per_s"] / out["fp64"]["trajectories_per_s"])
+        # mixed against FP64 on the first cmp_epochs epochs, both at cmp_step_limit: the
+        # FP64 per-lane TS05 tracer's trapped trajectories take minutes to reach 1e6 steps
+        ne, sl = min(args.cmp_epochs, args.epochs), args.cmp_step_limit
+        res = {}
+        for prec, name in ((A.PREC_MIXED, "mixed"), (A.PREC_FP64, "fp64")):
+            p = A.Plan(ms, params[:ne], rows, precision=prec, step_limit=sl, queue_order=1)
+            st, _ = timed_run(p, f"{name} ({ne} epochs, step limit {sl})")
+            res[name] = (st,) + p.results()
+            p.close()
+        (st_a, ta, ca), (st_d, td, cd) = res["mixed"], res["fp64"]
+        out["comparison"] = {"epochs": ne, "step_limit": sl, "mixed": rate_block(st_a, "ts05_igrf"),
+                             "fp64": rate_block(st_d, "ts05_igrf")}
+        out["comparison"]["speedup_mixed_vs_fp64"] = st_d.kernel_ms / st_a.kernel_ms
         per_scan = np.concatenate([rows] * ne)
-        out["mixed_vs_fp64"] = compare_cuts(cm[:ne], cd, tm[:ne], td, per_scan)
-        d = np.abs(cm[:ne]["rc"] - cd["rc"])
-        out["abs_dRc_GV_per_epoch_max"] = [float(v) for v in d.max(axis=1)]
-        out["abs_dRc_GV_per_site_max"] = [float(v) for v in d.max(axis=0)]
-    out["rc_range_GV"] = [float(cm["rc"].min()), float(cm["rc"].max())]
+        out["comparison"]["mixed_vs_fp64"] = compare_cuts(ca, cd, ta, td, per_scan)
+        d = np.abs(ca["rc"] - cd["rc"])
+        out["comparison"]["abs_dRc_GV_per_site_per_epoch"] = [[round(float(v), 3) for v in row] for row in d]
     return out
 
 
@@ -239,24 +243,30 @@ def run_map(args):
                 queue_order=tuned.get("queue_order", 1), waves_per_simd=tuned.get("waves_per_simd", 0),
                 block_threads=tuned.get("block_threads", 0))
     st_m, wall_m = timed_run(pm, "mixed")
-    tm, cm = pm.results()
+    _, cm = pm.results(traj=False)
     pm.close()
     out = {"workload": "1-degree global map, SH degree-20 core field (IGRF-2020-scaled) + TS05",
            "sites": len(rows), "nk1": args.nk1, "step_limit": args.step_limit, "autotuned": tuned,
            "mixed": rate_block(st_m, "ts05_sh20"), "mixed_wall_s": wall_m,
            "rc_range_GV": [float(cm["rc"].min()), float(cm["rc"].max())]}
     if not args.no_fp64:
+        # FP64 rerun on a seeded subsample of the sites, compared with mixed on the same
+        # sites, both at cmp_step_limit (the FP64 per-lane TS05 tracer's trapped
+        # trajectories take minutes to reach 1e6 steps)
         rng = np.random.default_rng(args.seed + 2)
         idx = np.sort(rng.choice(len(rows), max(1, int(round(args.fp64_frac * len(rows)))), replace=False))
-        pd = A.Plan(ms, params, rows[idx], precision=A.PREC_FP64, step_limit=args.step_limit, queue_order=1)
-        st_d, _ = timed_run(pd, f"fp64 ({len(idx)} sites)")
-        td, cd = pd.results()
-        pd.close()
-        per = 1000
-        tm_sub = np.concatenate([tm[i * per:(i + 1) * per] for i in idx])
-        out["fp64_subsample"] = rate_block(st_d, "ts05_sh20")
-        out["fp64_subsample"]["sites"] = int(len(idx))
-        out["mixed_vs_fp64"] = compare_cuts(cm[idx], cd, tm_sub, td, rows[idx])
+        sl = args.cmp_step_limit
+        res = {}
+        for prec, name in ((A.PREC_MIXED, "mixed"), (A.PREC_FP64, "fp64")):
+            p = A.Plan(ms, params, rows[idx], precision=prec, step_limit=sl, queue_order=1)
+            st, _ = timed_run(p, f"{name} ({len(idx)} sites, step limit {sl})")
+            res[name] = (st,) + p.results()
+            p.close()
+        (st_a, ta, ca), (st_d, td, cd) = res["mixed"], res["fp64"]
+        out["comparison"] = {"sites": int(len(idx)), "step_limit": sl, "mixed": rate_block(st_a, "ts05_sh20"),
+                             "fp64": rate_block(st_d, "ts05_sh20"),
+                             "speedup_mixed_vs_fp64": st_d.kernel_ms / st_a.kernel_ms,
+                             "mixed_vs_fp64": compare_cuts(ca, cd, ta, td, rows[idx])}
     return out
 
 
@@ -318,12 +328,14 @@ def main():
     s = sub.add_parser("storm")
     s.add_argument("--epochs", type=int, default=288)
     s.add_argument("--sites", type=int, default=50)
-    s.add_argument("--fp64-epochs", type=int, default=0, help="FP64 on the first N epochs (0: all)")
+    s.add_argument("--cmp-epochs", type=int, default=8, help="epochs of the mixed-vs-FP64 comparison")
+    s.add_argument("--cmp-step-limit", type=int, default=100000, help="step limit of the comparison runs")
     m = sub.add_parser("map")
     m.add_argument("--lat-step", type=float, default=1.0)
     m.add_argument("--fp64-frac", type=float, default=0.01)
     m.add_argument("--tune-sample", type=int, default=256)
     m.add_argument("--no-tune", action="store_true")
+    m.add_argument("--cmp-step-limit", type=int, default=100000, help="step limit of the FP64 comparison runs")
     d = sub.add_parser("directions")
     d.add_argument("--sites", type=int, default=64)
     d.add_argument("--fp64-scans", type=int, default=96)

@@ -95,6 +95,14 @@ typedef struct {
     int32_t block_threads;    /* threads per workgroup of the persistent grid: 0 = the library's
                                * choice (256, or 64 for launches with fewer trajectories than
                                * lanes), else 64 or 256.  Scheduling only */
+    int32_t handover_low_steps;  /* with the cooperative mode: the handover step count of the
+                                  * first handover_low_count rigidities of every scan (the
+                                  * lowest, where trapped trajectories concentrate); 0 = the
+                                  * library default (32768, or handover_steps if smaller),
+                                  * negative = handover_steps */
+    int32_t handover_low_count;  /* how many rigidities of each scan handover_low_steps covers:
+                                  * 0 = the library default (none), negative = none.  A property
+                                  * of the trajectory, so the results do not depend on scheduling */
 } cor_trace_opts;
 
 /* Per trajectory.  The doubles are set only for allowed trajectories: exit v/c, exit
