@@ -274,6 +274,8 @@ __global__ void reset_head_kernel(unsigned long long *head, CoopCtl *coop, RunSt
         coop->done = 0ull;
         coop->converting = 0;
         coop->feeders = 0;
+        coop->lres = 0ull;
+        coop->lhead = 0ull;
         coop->run = coop->run + 1u;   // this run's id for the park records
     }
 }
@@ -356,6 +358,7 @@ struct cor_trace_plan {
     uint32_t handover_low_steps; // ... for the lowest handover_low_count rigidities of a scan
     int32_t handover_low_count;
     size_t park_slots;
+    size_t pool_slots;           // per-lane pool records, after the park slots
     CoopCtl *d_coop;
     ParkDev *d_park;
     RunStats *d_run;
@@ -651,10 +654,13 @@ int plan_create_impl(const cor_model_set *models, const ModelsParams *epochs, in
         // the handover counters and the park slots: every trajectory parks at most once
         // from the per-lane mode, and workers hand back at most one per lane of the grid
         p->park_slots = (size_t)total + (size_t)p->grid * p->block;
+        // the per-lane pool after them: four records per lane of the grid per run (the
+        // compaction stops for the rest of a run that has used them up)
+        p->pool_slots = (size_t)p->grid * p->block * 4;
         PLAN_TRY(hipMalloc(&p->d_coop, sizeof(CoopCtl)));
-        PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * p->park_slots));
+        PLAN_TRY(hipMalloc(&p->d_park, sizeof(ParkDev) * (p->park_slots + p->pool_slots)));
         PLAN_TRY(hipMemsetAsync(p->d_coop, 0, sizeof(CoopCtl), g_stream));
-        PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * p->park_slots, g_stream));
+        PLAN_TRY(hipMemsetAsync(p->d_park, 0, sizeof(ParkDev) * (p->park_slots + p->pool_slots), g_stream));
         PLAN_TRY(hipStreamSynchronize(g_stream));
     }
     for (int k = 0; k < 3; k++) PLAN_TRY(hipEventCreate(&p->ev[k]));
@@ -779,6 +785,8 @@ hipError_t enqueue_run(cor_trace_plan *p, hipStream_t s, bool timing) {
     a.park = p->d_park;
     a.stats = p->d_run;
     a.repark_limit = (long long)p->grid * p->block;
+    a.pool = p->d_park ? p->d_park + p->park_slots : nullptr;
+    a.pool_cap = p->d_park ? (unsigned long long)p->pool_slots : 0ull;
     a.trk = p->d_trk;
     a.trk_off = p->d_trk_off;
     a.handover_steps = p->handover_steps;

@@ -115,6 +115,8 @@ struct CoopCtl {
     alignas(128) unsigned long long done;     // trajectories finished (both modes)
     alignas(128) int32_t converting;          // wavefronts running down their lanes to turn worker
     alignas(128) int32_t feeders;             // wavefronts taking new trajectories from the queue
+    alignas(128) unsigned long long lres;     // per-lane pool: records reserved by donating wavefronts
+    alignas(128) unsigned long long lhead;    // per-lane pool: records claimed by taking wavefronts
     alignas(128) uint32_t run;                // run id: advanced by every reset, never cleared, so
                                               // park records of earlier runs never match (a value
                                               // the device keeps, so a captured run replays right)
@@ -177,6 +179,9 @@ struct TraceArgs {
     ParkDev *park;
     RunStats *stats;
     long long repark_limit;      // park slots beyond the per-lane ones (one per lane of the grid)
+    ParkDev *pool;               // per-lane pool: trajectories of sparse per-lane wavefronts after
+                                 // the queue drained, continued per-lane by wavefronts with idle lanes
+    unsigned long long pool_cap; // records the pool holds over one run (0: no compaction)
     cor_track_point *trk;        // FP64 track recording (NULL: off)
     const int64_t *trk_off;      // first point of each trajectory (n_items + 1 entries)
     uint32_t handover_steps;     // per-lane accepted steps before the handover

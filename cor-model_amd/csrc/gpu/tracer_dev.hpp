@@ -706,7 +706,9 @@ __device__ bool coop_worker(const TraceArgs &a, uint32_t run_id, const EpochView
             if (!has_leads && !__any(holding)) {
                 // idle: with no parked trajectory waiting and the queue not yet drained,
                 // this wavefront goes back to taking new trajectories in the per-lane mode
-                if (head0 >= res0 && load_relaxed_uniform(a.head) < (unsigned long long)a.n_items) return false;
+                if (head0 >= res0 && (load_relaxed_uniform(a.head) < (unsigned long long)a.n_items ||
+                                      load_relaxed_uniform(&a.coop->lres) > load_relaxed_uniform(&a.coop->lhead)))
+                    return false;
                 want = 1;
                 wait_ok = true;
             } else if (free_leads && head0 < res0) {
@@ -871,6 +873,38 @@ __device__ bool coop_worker(const TraceArgs &a, uint32_t run_id, const EpochView
     }
 }
 
+// ---------------------------------------------------------------- per-lane compaction
+
+// After the queue is drained, a per-lane wavefront with at most POOL_DONATE_LANES live
+// trajectories puts them into the per-lane pool and turns worker; a wavefront with idle
+// lanes takes pooled trajectories into them (idle workers come back to the per-lane mode
+// for them).  A pooled trajectory continues in the per-lane mode, with the same
+// arithmetic, so compaction changes no result.  A wavefront that took from the pool
+// runs POOL_HOLD_ITERS steps before it may give to it again.
+constexpr int POOL_DONATE_LANES = 16;
+constexpr unsigned POOL_HOLD_ITERS = 256;
+
+__device__ __forceinline__ void pool_write(ParkDev *P, uint32_t run_id, int64_t gid, v3<double> pos, v3<double> vel,
+                                           double v, double time, double alength, double zn, unsigned int step,
+                                           uint32_t attempts, int nk1, uint32_t ho_at) {
+    P->pos[0] = pos.x;
+    P->pos[1] = pos.y;
+    P->pos[2] = pos.z;
+    P->vel[0] = vel.x;
+    P->vel[1] = vel.y;
+    P->vel[2] = vel.z;
+    P->v = v;
+    P->time = time;
+    P->alength = alength;
+    P->zn = zn;
+    P->gid = gid;
+    P->step = step;
+    P->attempts = attempts;
+    P->nk1 = nk1;
+    P->handover = ho_at;          // the trajectory's handover step count travels with it
+    __hip_atomic_store(&P->ready, run_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------- the kernel
 
 // Register budget: COR_WAVES_PER_EU wavefronts per SIMD (set per translation unit; the
@@ -924,6 +958,7 @@ trace_kernel(TraceArgs a) {
     uint32_t attempts = 0;
     uint32_t ho_at = 0xffffffffu;   // this trajectory's handover step (cooperative mode)
     unsigned iter = 0;
+    unsigned no_donate_until = 0;   // per-lane pool: iteration before which this wavefront keeps its lanes
     // fill of this wavefront's per-lane loop
     unsigned long long wave_iters = 0, lane_iters = 0, wave_iters_open = 0, lane_iters_open = 0;
     unsigned long long ended_here = 0;
@@ -1028,6 +1063,58 @@ trace_kernel(TraceArgs a) {
                                 feeding = false;
                                 converting = true;
                             }
+                        }
+                    }
+                }
+            }
+            if constexpr (kCoop) {
+                // ---- 3b. per-lane compaction once this wavefront takes no new trajectories
+                const uint64_t live = __ballot(active);
+                if (a.coop_on && a.pool_cap > 0 && !feeding && ((iter & 15u) == 0 || !live)) {
+                    const int nl = __popcll(live);
+                    const unsigned long long ph = load_relaxed_uniform(&a.coop->lhead);
+                    const unsigned long long pr = load_relaxed_uniform(&a.coop->lres);
+                    if (pr > ph && nl < 64) {
+                        // take waiting records into the idle lanes (claimed by compare-and-swap;
+                        // each is published by its donor right after the reservation)
+                        const int want = (int)min((unsigned long long)(64 - nl), pr - ph);
+                        unsigned long long base = ~0ull;
+                        if (lane == 0 && atomicCAS(&a.coop->lhead, ph, ph + (unsigned long long)want) == ph) base = ph;
+                        base = __shfl(base, 0);
+                        if (base != ~0ull) {
+                            const int rank = __popcll(~live & ((1ull << lane) - 1ull));
+                            if (!active && rank < want) {
+                                ParkDev *P = a.pool + base + rank;
+                                while (__hip_atomic_load(&P->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != run_id)
+                                    __builtin_amdgcn_s_sleep(1);
+                                pos = mk3(P->pos[0], P->pos[1], P->pos[2]);
+                                vel = mk3(P->vel[0], P->vel[1], P->vel[2]);
+                                v = P->v;
+                                time = P->time;
+                                alength = P->alength;
+                                zn = P->zn;
+                                gid = P->gid;
+                                step = P->step;
+                                attempts = P->attempts;
+                                nk1 = P->nk1;
+                                nk0 = nk1;          // a step's top: nk0 is back at nk1
+                                ho_at = P->handover;
+                                active = true;
+                            }
+                            no_donate_until = iter + POOL_HOLD_ITERS;
+                        }
+                    } else if (nl > 0 && nl <= POOL_DONATE_LANES && iter >= no_donate_until) {
+                        // give the live trajectories to the pool (while it has room) and turn worker
+                        unsigned long long base = ~0ull;
+                        if (lane == 0 && pr + (unsigned long long)nl <= a.pool_cap &&
+                            atomicCAS(&a.coop->lres, pr, pr + (unsigned long long)nl) == pr)
+                            base = pr;
+                        base = __shfl(base, 0);
+                        if (base != ~0ull) {
+                            if (active)
+                                pool_write(a.pool + base + __popcll(live & ((1ull << lane) - 1ull)), run_id, gid, pos,
+                                           vel, v, time, alength, zn, step, attempts, nk1, ho_at);
+                            active = false;
                         }
                     }
                 }

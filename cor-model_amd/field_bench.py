@@ -7,7 +7,7 @@ twice.  Kernel times come from a kernel trace of this script, e.g.
 
 The batches: FP64 and mixed one point per lane (cor::field_batch_kernel<0|1, 1, 1, 0>),
 and the cooperative evaluator with 64, 32, 16 and 8 lanes per point
-(cor::field_coop_kernel<1, 1, 0, GS>), in that order."""
+(cor::field_coop_kernel<1, 1, 1, 0, GS>), in that order."""
 import sys
 from pathlib import Path
 

@@ -113,13 +113,31 @@ def timed_run(plan, label, every=20.0):
     return st, wall
 
 
-def rate_block(st, flops_key):
+def axis_ended(tr):
+    """Trajectories whose state turned NaN (a start exactly on the polar axis, where the
+    SH synthesis divides by sin(theta) = 0, as the reference's does): they end at once
+    and count the step limit as their steps, by the reference's convention, but
+    compute only their attempts."""
+    return (tr["status"] == 2) & (tr["attempts"] < tr["steps"])
+
+
+def rate_block(st, flops_key, tr=None):
+    """Rates over the kernel's time.  With the per-trajectory records, the steps that
+    were computed: the NaN-ended trajectories count their attempts, not the step limit."""
     k = st.kernel_ms * 1e-3
+    steps = int(st.steps)
     out = {"trajectories": int(st.n_traj), "kernel_s": k, "trajectories_per_s": st.n_traj / k,
-           "steps": int(st.steps), "steps_per_s": st.steps / k, "attempts": int(st.attempts)}
+           "steps": steps, "attempts": int(st.attempts)}
+    if tr is not None:
+        nan = axis_ended(tr)
+        if nan.any():
+            steps = int(tr["steps"][~nan].sum(dtype=np.int64) + tr["attempts"][nan].sum(dtype=np.int64))
+            out["nan_ended_trajectories"] = int(nan.sum())
+            out["steps_computed"] = steps
+    out["steps_per_s"] = steps / k
     f = flop_model(flops_key) if flops_key else None
     if f is not None:
-        ach = f * st.steps / k / 1e12
+        ach = f * steps / k / 1e12
         out["roofline"] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s",
                            "frac": ach / PEAK_FP32, "flops_per_step": f, "flop_model": flops_key}
     return out
@@ -243,12 +261,16 @@ def run_map(args):
                 queue_order=tuned.get("queue_order", 1), waves_per_simd=tuned.get("waves_per_simd", 0),
                 block_threads=tuned.get("block_threads", 0))
     st_m, wall_m = timed_run(pm, "mixed")
-    _, cm = pm.results(traj=False)
+    tm, cm = pm.results()
     pm.close()
-    out = {"workload": "1-degree global map, SH degree-20 core field (IGRF-2020-scaled) + TS05",
+    per = 1000
+    axis_sites = np.unique(np.nonzero(axis_ended(tm))[0] // per)
+    out = {"workload": f"{args.lat_step:g}-degree global map, SH degree-20 core field (IGRF-2020-scaled) + TS05",
            "sites": len(rows), "nk1": args.nk1, "step_limit": args.step_limit, "autotuned": tuned,
-           "mixed": rate_block(st_m, "ts05_sh20"), "mixed_wall_s": wall_m,
+           "mixed": rate_block(st_m, "ts05_sh20", tm), "mixed_wall_s": wall_m,
+           "axis_sites": [int(k) for k in axis_sites],
            "rc_range_GV": [float(cm["rc"].min()), float(cm["rc"].max())]}
+    del tm
     if not args.no_fp64:
         # FP64 rerun on a seeded subsample of the sites, compared with mixed on the same
         # sites, both at cmp_step_limit (the FP64 per-lane TS05 tracer's trapped
@@ -263,10 +285,19 @@ def run_map(args):
             res[name] = (st,) + p.results()
             p.close()
         (st_a, ta, ca), (st_d, td, cd) = res["mixed"], res["fp64"]
-        out["comparison"] = {"sites": int(len(idx)), "step_limit": sl, "mixed": rate_block(st_a, "ts05_sh20"),
-                             "fp64": rate_block(st_d, "ts05_sh20"),
+        # scans starting on the polar axis, where the SH synthesis divides by sin(colatitude)
+        # = 0 and a trajectory whose state turns NaN ends at once (the reference's
+        # convention), in either precision: they are reported apart
+        nan_d = axis_ended(td).reshape(len(idx), -1).any(axis=1)
+        nan_a = axis_ended(ta).reshape(len(idx), -1).any(axis=1)
+        keep = ~(nan_d | nan_a)
+        out["comparison"] = {"sites": int(len(idx)), "step_limit": sl, "mixed": rate_block(st_a, "ts05_sh20", ta),
+                             "fp64": rate_block(st_d, "ts05_sh20", td),
                              "speedup_mixed_vs_fp64": st_d.kernel_ms / st_a.kernel_ms,
-                             "mixed_vs_fp64": compare_cuts(ca, cd, ta, td, rows[idx])}
+                             "axis_sites_excluded": int((~keep).sum()),
+                             "mixed_vs_fp64": compare_cuts(ca[keep], cd[keep],
+                                                           ta.reshape(len(idx), -1)[keep],
+                                                           td.reshape(len(idx), -1)[keep], rows[idx][keep])}
     return out
 
 

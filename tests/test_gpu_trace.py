@@ -223,6 +223,27 @@ def test_mixed_default_is_deterministic(cor):
         assert np.array_equal(cut, c0), (hs, q)
 
 
+def test_per_lane_compaction_changes_no_result(cor):
+    """With the cooperative mode on but a handover step no trajectory reaches, every
+    trajectory stays in the per-lane mode, and after the queue drains the sparse
+    per-lane wavefronts hand their trajectories to the per-lane pool, from which
+    wavefronts with idle lanes (and idle workers coming back) continue them.  The
+    records and cutoffs equal the per-lane-only run bit for bit, in FP64 and mixed
+    precision, and a second compacting run equals the first."""
+    sites = cor.grid_sites()[::7]
+    rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
+    p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
+    ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
+    for prec, coop in ((cor.PREC_MIXED, 0), (cor.PREC_FP64, 1)):
+        t0, c0 = cor.trace(ms, p, rows, step_limit=100000, precision=prec, coop=-1)
+        for _ in range(2):
+            t1, c1 = cor.trace(ms, p, rows, step_limit=100000, precision=prec, coop=coop, handover_steps=2000000000)
+            assert (t1["handover"] == 0).all()
+            assert np.array_equal(t1, t0), (prec, int((t1 != t0).sum()))
+            assert np.array_equal(c1, c0), prec
+        print(len(rows), "scans, precision", prec, ": compacting runs equal the per-lane-only run")
+
+
 TS05_STORM = [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5]
 
 
