@@ -256,6 +256,16 @@ def run_map(args):
     sites = grid_sites_step(args.lat_step)
     rows = vertical_rows(sites, 0.02, 0.02, 20.0, args.nk1)
     log(f"map: {len(rows)} sites x 1000 rigidities")
+    out = {"workload": f"{args.lat_step:g}-degree global map, SH degree-20 core field (IGRF-2020-scaled) + TS05",
+           "sites": len(rows), "nk1": args.nk1, "step_limit": args.step_limit}
+    if not args.cmp_only:
+        out.update(map_full(ms, params, rows, args))
+    if not args.no_fp64:
+        out.update(map_comparison(ms, params, rows, args))
+    return out
+
+
+def map_full(ms, params, rows, args):
     tuned = autotune(ms, params, rows, args) if not args.no_tune else {}
     pm = A.Plan(ms, params, rows, precision=A.PREC_MIXED, step_limit=args.step_limit,
                 queue_order=tuned.get("queue_order", 1), waves_per_simd=tuned.get("waves_per_simd", 0),
@@ -265,39 +275,39 @@ def run_map(args):
     pm.close()
     per = 1000
     axis_sites = np.unique(np.nonzero(axis_ended(tm))[0] // per)
-    out = {"workload": f"{args.lat_step:g}-degree global map, SH degree-20 core field (IGRF-2020-scaled) + TS05",
-           "sites": len(rows), "nk1": args.nk1, "step_limit": args.step_limit, "autotuned": tuned,
-           "mixed": rate_block(st_m, "ts05_sh20", tm), "mixed_wall_s": wall_m,
-           "axis_sites": [int(k) for k in axis_sites],
-           "rc_range_GV": [float(cm["rc"].min()), float(cm["rc"].max())]}
-    del tm
-    if not args.no_fp64:
-        # FP64 rerun on a seeded subsample of the sites, compared with mixed on the same
-        # sites, both at cmp_step_limit (the FP64 per-lane TS05 tracer's trapped
-        # trajectories take minutes to reach 1e6 steps)
-        rng = np.random.default_rng(args.seed + 2)
-        idx = np.sort(rng.choice(len(rows), max(1, int(round(args.fp64_frac * len(rows)))), replace=False))
-        sl = args.cmp_step_limit
-        res = {}
-        for prec, name in ((A.PREC_MIXED, "mixed"), (A.PREC_FP64, "fp64")):
-            p = A.Plan(ms, params, rows[idx], precision=prec, step_limit=sl, queue_order=1)
-            st, _ = timed_run(p, f"{name} ({len(idx)} sites, step limit {sl})")
-            res[name] = (st,) + p.results()
-            p.close()
-        (st_a, ta, ca), (st_d, td, cd) = res["mixed"], res["fp64"]
-        # scans starting on the polar axis, where the SH synthesis divides by sin(colatitude)
-        # = 0 and a trajectory whose state turns NaN ends at once (the reference's
-        # convention), in either precision: they are reported apart
-        nan_d = axis_ended(td).reshape(len(idx), -1).any(axis=1)
-        nan_a = axis_ended(ta).reshape(len(idx), -1).any(axis=1)
-        keep = ~(nan_d | nan_a)
-        out["comparison"] = {"sites": int(len(idx)), "step_limit": sl, "mixed": rate_block(st_a, "ts05_sh20", ta),
-                             "fp64": rate_block(st_d, "ts05_sh20", td),
-                             "speedup_mixed_vs_fp64": st_d.kernel_ms / st_a.kernel_ms,
-                             "axis_sites_excluded": int((~keep).sum()),
-                             "mixed_vs_fp64": compare_cuts(ca[keep], cd[keep],
-                                                           ta.reshape(len(idx), -1)[keep],
-                                                           td.reshape(len(idx), -1)[keep], rows[idx][keep])}
+    out = {"autotuned": tuned, "mixed": rate_block(st_m, "ts05_sh20", tm), "mixed_wall_s": wall_m,
+           "axis_sites": [int(k) for k in axis_sites], "rc_range_GV": [float(cm["rc"].min()), float(cm["rc"].max())]}
+    return out
+
+
+def map_comparison(ms, params, rows, args):
+    """FP64 rerun on a seeded subsample of the sites, compared with mixed on the same
+    sites, both at cmp_step_limit (the FP64 per-lane TS05 tracer's trapped trajectories
+    take minutes to reach 1e6 steps)."""
+    out = {}
+    rng = np.random.default_rng(args.seed + 2)
+    idx = np.sort(rng.choice(len(rows), max(1, int(round(args.fp64_frac * len(rows)))), replace=False))
+    sl = args.cmp_step_limit
+    res = {}
+    for prec, name in ((A.PREC_MIXED, "mixed"), (A.PREC_FP64, "fp64")):
+        p = A.Plan(ms, params, rows[idx], precision=prec, step_limit=sl, queue_order=1)
+        st, _ = timed_run(p, f"{name} ({len(idx)} sites, step limit {sl})")
+        res[name] = (st,) + p.results()
+        p.close()
+    (st_a, ta, ca), (st_d, td, cd) = res["mixed"], res["fp64"]
+    # scans starting on the polar axis, where the SH synthesis divides by sin(colatitude)
+    # = 0 and a trajectory whose state turns NaN ends at once (the reference's
+    # convention), in either precision: they are reported apart
+    nan_d = axis_ended(td).reshape(len(idx), -1).any(axis=1)
+    nan_a = axis_ended(ta).reshape(len(idx), -1).any(axis=1)
+    keep = ~(nan_d | nan_a)
+    out["comparison"] = {"sites": int(len(idx)), "step_limit": sl, "mixed": rate_block(st_a, "ts05_sh20", ta),
+                         "fp64": rate_block(st_d, "ts05_sh20", td),
+                         "speedup_mixed_vs_fp64": st_d.kernel_ms / st_a.kernel_ms,
+                         "axis_sites_excluded": int((~keep).sum()),
+                         "mixed_vs_fp64": compare_cuts(ca[keep], cd[keep],
+                                                       ta.reshape(len(idx), -1)[keep],
+                                                       td.reshape(len(idx), -1)[keep], rows[idx][keep])}
     return out
 
 
@@ -367,6 +377,7 @@ def main():
     m.add_argument("--tune-sample", type=int, default=256)
     m.add_argument("--no-tune", action="store_true")
     m.add_argument("--cmp-step-limit", type=int, default=100000, help="step limit of the FP64 comparison runs")
+    m.add_argument("--cmp-only", action="store_true", help="only the mixed-vs-FP64 comparison on the subsample")
     d = sub.add_parser("directions")
     d.add_argument("--sites", type=int, default=64)
     d.add_argument("--fp64-scans", type=int, default=96)

@@ -234,10 +234,11 @@ def test_per_lane_compaction_changes_no_result(cor):
     rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
-    for prec, coop in ((cor.PREC_MIXED, 0), (cor.PREC_FP64, 1)):
-        t0, c0 = cor.trace(ms, p, rows, step_limit=100000, precision=prec, coop=-1)
-        for _ in range(2):
-            t1, c1 = cor.trace(ms, p, rows, step_limit=100000, precision=prec, coop=coop, handover_steps=2000000000)
+    for prec, coop, reps in ((cor.PREC_MIXED, 0, 2), (cor.PREC_FP64, 1, 1)):
+        t0, c0 = cor.trace(ms, p, rows, step_limit=100000 if prec else 30000, precision=prec, coop=-1)
+        for _ in range(reps):
+            t1, c1 = cor.trace(ms, p, rows, step_limit=100000 if prec else 30000, precision=prec, coop=coop,
+                               handover_steps=2000000000)
             assert (t1["handover"] == 0).all()
             assert np.array_equal(t1, t0), (prec, int((t1 != t0).sum()))
             assert np.array_equal(c1, c0), prec
