@@ -314,7 +314,22 @@ __device__ __forceinline__ void fialcos1(T r, T th, T ph, T th0, T dt, T tgp, T 
     Dm<T>::sincos(th, &st, &ct);
     Dm<T>::sincos(ph, &sp, &cp);
     const T ro = r * st;
-    const T tg = st / (T(1.) + ct), ctg = st / (T(1.) - ct);
+    T tg, ctg;   // tan(th/2), cot(th/2)
+    if constexpr (Dm<T>::fp64) {
+        tg = st / (T(1.) + ct);
+        ctg = st / (T(1.) - ct);
+    } else {
+        // the same half-angle tangents in the form without the cancellation of 1 -+ cos th:
+        // in FP32, cos th rounds to +-1 within ~3e-4 rad of the cone's axis, where the
+        // reference's form divides by zero and the field would turn NaN
+        if (ct >= T(0.)) {
+            tg = st / (T(1.) + ct);
+            ctg = (T(1.) + ct) / st;
+        } else {
+            tg = (T(1.) - ct) / st;
+            ctg = st / (T(1.) - ct);
+        }
+    }
     const T tnp = th0 + dt, tnm = th0 - dt;
     T tgm2 = 0, tgp2 = 0;
     if (th >= tnm) {

@@ -147,8 +147,20 @@ def test_cooperative_field_fp64(cor, oracle, name, kinds, parmod, pts):
     assert err.max() < 1e-10
 
 
+def cone_axis_points():
+    """Nightside points within 0.002 Re of the GSM y = 0 plane where TS05's Region-1/2
+    cone coordinates (lib/ts05/birk_tot.c:252-377) put the point within ~3e-4 rad of the
+    cone's axis: three points where a trajectory of the 1-degree map's workload met it,
+    and for each a line of 401 points across y = 0."""
+    base = [(-7.454434, -15.121027), (-4.367026, -9.75418), (-3.81447, -8.748972)]
+    ys = np.linspace(-0.002, 0.002, 401)
+    return np.array([(x, y, z) for (x, z) in base for y in ys]) * RE
+
+
 TS05_CASES = [
     ("ts05", dict(internal=0, external=2), [2, -20, 0, -2, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], box_points(6, 4000)),
+    ("ts05 cone axis", dict(internal=0, external=2), [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5],
+     cone_axis_points()),
     ("igrf+ts05 storm", dict(internal=1, external=2), [3, -100, 2, -5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5],
      box_points(7, 4000)),
     ("igrf+ts05 drivers", dict(internal=1, external=2), [8, -250, -4, 6, 1.5, 0.2, 1.9, 0.7, 1.1, 0.3],
