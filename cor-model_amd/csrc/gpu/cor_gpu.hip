@@ -206,9 +206,7 @@ struct ScanStat {
 };
 
 __global__ void __launch_bounds__(64) penumbra_kernel(const ScanDev *__restrict__ scans,
-                                                      const int32_t *__restrict__ status,
-                                                      const uint32_t *__restrict__ steps,
-                                                      const uint32_t *__restrict__ attempts,
+                                                      const cor_traj *__restrict__ rec,
                                                       cor_cutoff *__restrict__ cut,
                                                       ScanStat *__restrict__ stat) {
     const ScanDev &S = scans[blockIdx.x];
@@ -217,9 +215,10 @@ __global__ void __launch_bounds__(64) penumbra_kernel(const ScanDev *__restrict_
     unsigned long long st = 0, at = 0;
     for (int i = lane; i < S.n; i += 64) {
         const int64_t g = S.first + i;
-        const int s = status[g];
-        st += steps[g];
-        at += attempts[g];
+        const uint4 h = *reinterpret_cast<const uint4 *>(rec + g);   // status, steps, attempts, handover
+        const int s = (int)h.x;
+        st += h.y;
+        at += h.z;
         if (s == ST_ALLOWED) {
             imin = min(imin, i);
         } else {
@@ -347,7 +346,7 @@ struct cor_trace_plan {
     int64_t n_items;
     uint32_t *d_order;
     unsigned long long *d_head;
-    TraceOut out;
+    cor_traj *out = nullptr;        // result records (one per trajectory)
     cor_cutoff *d_cut;
     ScanStat *d_stat;
     uint64_t step_limit;
@@ -451,11 +450,7 @@ void cor_trace_plan_destroy(cor_trace_plan *p) {
     (void)hipFree(p->d_scans);
     (void)hipFree(p->d_order);
     (void)hipFree(p->d_head);
-    (void)hipFree(p->out.status);
-    (void)hipFree(p->out.steps);
-    (void)hipFree(p->out.attempts);
-    (void)hipFree(p->out.handover);
-    (void)hipFree(p->out.vc);
+    (void)hipFree(p->out);
     (void)hipFree(p->d_cut);
     (void)hipFree(p->d_stat);
     (void)hipFree(p->d_coop);
@@ -579,20 +574,8 @@ int plan_create_impl(const cor_model_set *models, const ModelsParams *epochs, in
     PLAN_TRY(hipMalloc(&p->d_scans, sizeof(ScanDev) * n_scans));
     PLAN_TRY(hipMemcpy(p->d_scans, p->h_scans.data(), sizeof(ScanDev) * n_scans, hipMemcpyHostToDevice));
     PLAN_TRY(hipMalloc(&p->d_head, sizeof(unsigned long long)));
-    PLAN_TRY(hipMalloc(&p->out.status, sizeof(int32_t) * nt));
-    PLAN_TRY(hipMalloc(&p->out.steps, sizeof(uint32_t) * nt));
-    PLAN_TRY(hipMalloc(&p->out.attempts, sizeof(uint32_t) * nt));
-    PLAN_TRY(hipMalloc(&p->out.handover, sizeof(uint32_t) * nt));
-    // the eight per-trajectory doubles share one allocation, one array after another
-    PLAN_TRY(hipMalloc(&p->out.vc, sizeof(double) * nt * 8));
-    p->out.r = p->out.vc + nt;
-    p->out.td = p->out.r + nt;
-    p->out.fei = p->out.td + nt;
-    p->out.ast = p->out.fei + nt;
-    p->out.asf = p->out.ast + nt;
-    p->out.time = p->out.asf + nt;
-    p->out.length = p->out.time + nt;
-    PLAN_TRY(hipMemset(p->out.vc, 0, sizeof(double) * nt * 8));
+    PLAN_TRY(hipMalloc(&p->out, sizeof(cor_traj) * nt));
+    PLAN_TRY(hipMemset(p->out, 0, sizeof(cor_traj) * nt));
     PLAN_TRY(hipMalloc(&p->d_cut, sizeof(cor_cutoff) * n_scans));
     PLAN_TRY(hipMalloc(&p->d_stat, sizeof(ScanStat) * n_scans));
     PLAN_TRY(hipMalloc(&p->d_run, sizeof(RunStats)));
@@ -809,7 +792,7 @@ hipError_t enqueue_run(cor_trace_plan *p, hipStream_t s, bool timing) {
         if ((e = hipLaunchKernel(p->kernel, dim3(p->grid), dim3(p->block), args, 0, s)) != hipSuccess) return e;
     }
     if (timing && (e = hipEventRecord(p->ev[1], s)) != hipSuccess) return e;
-    penumbra_kernel<<<p->n_scans, 64, 0, s>>>(p->d_scans, p->out.status, p->out.steps, p->out.attempts,
+    penumbra_kernel<<<p->n_scans, 64, 0, s>>>(p->d_scans, p->out,
                                               p->d_cut, p->d_stat);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (timing && (e = hipEventRecord(p->ev[2], s)) != hipSuccess) return e;
@@ -862,35 +845,8 @@ int cor_trace_plan_results(cor_trace_plan *p, cor_traj *traj, cor_cutoff *cut) {
         }
     }
     if (cut) HIP_TRY(hipMemcpy(cut, p->d_cut, sizeof(cor_cutoff) * p->n_scans, hipMemcpyDeviceToHost));
-    if (traj && p->n_items > 0) {
-        const size_t n = (size_t)p->n_items;
-        std::vector<int32_t> st(n);
-        std::vector<uint32_t> sp(n), at(n), ho(n);
-        std::vector<double> d(n * 8);
-        HIP_TRY(hipMemcpy(st.data(), p->out.status, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(sp.data(), p->out.steps, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(at.data(), p->out.attempts, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(ho.data(), p->out.handover, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-        // the eight arrays were allocated as one block of n_items-strided arrays
-        const size_t nt = (size_t)std::max<int64_t>(p->n_items, 1);
-        std::vector<double> blk(nt * 8);
-        HIP_TRY(hipMemcpy(blk.data(), p->out.vc, sizeof(double) * nt * 8, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n; i++) {
-            cor_traj &t = traj[i];
-            t.status = st[i];
-            t.steps = sp[i];
-            t.attempts = at[i];
-            t.handover = ho[i];
-            t.vc = blk[i];
-            t.r = blk[nt + i];
-            t.td = blk[2 * nt + i];
-            t.fei = blk[3 * nt + i];
-            t.ast = blk[4 * nt + i];
-            t.asf = blk[5 * nt + i];
-            t.time = blk[6 * nt + i];
-            t.length_m = blk[7 * nt + i];
-        }
-    }
+    if (traj && p->n_items > 0)
+        HIP_TRY(hipMemcpy(traj, p->out, sizeof(cor_traj) * (size_t)p->n_items, hipMemcpyDeviceToHost));
     return COR_OK;
 }
 
@@ -958,21 +914,15 @@ constexpr int SERIES_MAX_IN_FLIGHT = 4;
 hipStream_t g_series_stream[SERIES_MAX_IN_FLIGHT] = {};
 
 // Offsets of the pinned result block of an epoch graph: the CoopCtl counters, the
-// per-scan cutoffs, then status, steps, attempts, handover and the eight doubles per
-// trajectory (the device's one-array-after-another layout).
+// per-scan cutoffs, then the trajectories' cor_traj records.
 struct ResLayout {
-    size_t cut, status, steps, attempts, handover, dbl, total;
+    size_t cut, rec, total;
 };
 ResLayout res_layout(const cor_trace_plan *p) {
-    const size_t n = (size_t)p->n_items, nt = (size_t)std::max<int64_t>(p->n_items, 1);
     ResLayout L;
     L.cut = sizeof(CoopCtl);
-    L.status = L.cut + sizeof(cor_cutoff) * (size_t)p->n_scans;
-    L.steps = L.status + sizeof(int32_t) * n;
-    L.attempts = L.steps + sizeof(uint32_t) * n;
-    L.handover = L.attempts + sizeof(uint32_t) * n;
-    L.dbl = (L.handover + sizeof(uint32_t) * n + 7) & ~(size_t)7;
-    L.total = L.dbl + sizeof(double) * 8 * nt;
+    L.rec = (L.cut + sizeof(cor_cutoff) * (size_t)p->n_scans + 15) & ~(size_t)15;
+    L.total = L.rec + sizeof(cor_traj) * (size_t)std::max<int64_t>(p->n_items, 1);
     return L;
 }
 
@@ -989,7 +939,7 @@ hipError_t capture_epoch_graph(cor_trace_plan *p, hipStream_t s) {
     if ((e = hipHostMalloc((void **)&p->h_res, L.total, hipHostMallocDefault)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&p->ev_graph, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
-    const size_t n = (size_t)p->n_items, nt = (size_t)std::max<int64_t>(p->n_items, 1);
+    const size_t n = (size_t)p->n_items;
     e = hipMemcpyAsync(p->d_epoch, p->h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s);
     if (e == hipSuccess && (p->coop_on || needs_coop_tables(p->prec, p->ext))) e = build_coop_tables(p->d_epoch, s);
     if (e == hipSuccess) e = enqueue_run(p, s, false);
@@ -997,17 +947,8 @@ hipError_t capture_epoch_graph(cor_trace_plan *p, hipStream_t s) {
         e = hipMemcpyAsync(p->h_res, p->d_coop, sizeof(CoopCtl), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(p->h_res + L.cut, p->d_cut, sizeof(cor_cutoff) * p->n_scans, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && n > 0) {
-        e = hipMemcpyAsync(p->h_res + L.status, p->out.status, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(p->h_res + L.steps, p->out.steps, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(p->h_res + L.attempts, p->out.attempts, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(p->h_res + L.handover, p->out.handover, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(p->h_res + L.dbl, p->out.vc, sizeof(double) * 8 * nt, hipMemcpyDeviceToHost, s);
-    }
+    if (e == hipSuccess && n > 0)
+        e = hipMemcpyAsync(p->h_res + L.rec, p->out, sizeof(cor_traj) * n, hipMemcpyDeviceToHost, s);
     // always leave capture mode, also after a failed enqueue
     hipGraph_t g = nullptr;
     const hipError_t e_end = hipStreamEndCapture(s, &g);
@@ -1033,29 +974,7 @@ int collect_graph(cor_trace_plan *p, cor_traj *traj, cor_cutoff *cut) {
         }
     }
     if (cut) memcpy(cut, p->h_res + L.cut, sizeof(cor_cutoff) * p->n_scans);
-    if (traj) {
-        const size_t n = (size_t)p->n_items, nt = (size_t)std::max<int64_t>(p->n_items, 1);
-        const int32_t *st = (const int32_t *)(p->h_res + L.status);
-        const uint32_t *sp = (const uint32_t *)(p->h_res + L.steps);
-        const uint32_t *at = (const uint32_t *)(p->h_res + L.attempts);
-        const uint32_t *ho = (const uint32_t *)(p->h_res + L.handover);
-        const double *blk = (const double *)(p->h_res + L.dbl);
-        for (size_t i = 0; i < n; i++) {
-            cor_traj &t = traj[i];
-            t.status = st[i];
-            t.steps = sp[i];
-            t.attempts = at[i];
-            t.handover = ho[i];
-            t.vc = blk[i];
-            t.r = blk[nt + i];
-            t.td = blk[2 * nt + i];
-            t.fei = blk[3 * nt + i];
-            t.ast = blk[4 * nt + i];
-            t.asf = blk[5 * nt + i];
-            t.time = blk[6 * nt + i];
-            t.length_m = blk[7 * nt + i];
-        }
-    }
+    if (traj && p->n_items > 0) memcpy(traj, p->h_res + L.rec, sizeof(cor_traj) * (size_t)p->n_items);
     return COR_OK;
 }
 }  // namespace
@@ -1119,9 +1038,13 @@ int cor_gpu_trace(const cor_model_set *models, const ModelsParams *epoch, const 
 }
 
 int cor_gpu_trace_noalloc(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
-                          int32_t n_scans, const cor_trace_opts *opts, void *d_traj, cor_cutoff *d_cut,
+                          int32_t n_scans, const cor_trace_opts *opts, cor_traj *d_traj, cor_cutoff *d_cut,
                           void *stream) {
     if (!epoch || !opts || !scans || !d_traj || !d_cut) return COR_E_INVALID;
+    if (((uintptr_t)d_traj & 15u) != 0) {
+        snprintf(g_hip_error, sizeof g_hip_error, "cor_gpu_trace_noalloc: d_traj must be 16-byte aligned");
+        return COR_E_INVALID;
+    }
     if (n_scans < 1 || n_scans > COR_NOALLOC_MAX_SCANS) {
         snprintf(g_hip_error, sizeof g_hip_error, "cor_gpu_trace_noalloc takes 1 to %d scans, not %d",
                  COR_NOALLOC_MAX_SCANS, (int)n_scans);
@@ -1157,24 +1080,9 @@ int cor_gpu_trace_noalloc(const cor_model_set *models, const ModelsParams *epoch
     HIP_TRY(hipMemcpyAsync(&sc->epoch, &h_epoch, sizeof(EpochDev), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(sc->scans, h_scans, sizeof(ScanDev) * n_scans, hipMemcpyHostToDevice, s));
     if (needs_coop_tables(opts->precision, ext)) HIP_TRY(build_coop_tables(&sc->epoch, s));
-    // the caller's record block: status, steps, attempts, handover (4 bytes each), then
-    // the eight doubles, each an n-strided array
-    const size_t n = (size_t)total;
-    unsigned char *base = (unsigned char *)d_traj;
     TraceArgs a;
     memset(&a, 0, sizeof a);
-    a.out.status = (int32_t *)base;
-    a.out.steps = (uint32_t *)(base + 4 * n);
-    a.out.attempts = (uint32_t *)(base + 8 * n);
-    a.out.handover = (uint32_t *)(base + 12 * n);
-    a.out.vc = (double *)(base + 16 * n);
-    a.out.r = a.out.vc + n;
-    a.out.td = a.out.r + n;
-    a.out.fei = a.out.td + n;
-    a.out.ast = a.out.fei + n;
-    a.out.asf = a.out.ast + n;
-    a.out.time = a.out.asf + n;
-    a.out.length = a.out.time + n;
+    a.out = d_traj;                        // the caller's records, one per trajectory
     a.epoch = &sc->epoch;
     a.scans = sc->scans;
     a.n_scans = n_scans;
@@ -1204,7 +1112,7 @@ int cor_gpu_trace_noalloc(const cor_model_set *models, const ModelsParams *epoch
         void *args[] = {&a};
         HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(block), args, 0, s));
     }
-    penumbra_kernel<<<n_scans, 64, 0, s>>>(sc->scans, a.out.status, a.out.steps, a.out.attempts, d_cut, sc->stat);
+    penumbra_kernel<<<n_scans, 64, 0, s>>>(sc->scans, a.out, d_cut, sc->stat);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return COR_OK;

@@ -11,6 +11,7 @@
 //   results    structure of arrays indexed by global trajectory id (coalesced-free,
 //              written once per trajectory).
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 #include "cor_gpu.h"
 #include "t96_dev.hpp"
@@ -132,13 +133,10 @@ struct ScanDev {
     int32_t pad;
 };
 
-struct TraceOut {
-    int32_t *status;
-    uint32_t *steps;
-    uint32_t *attempts;
-    uint32_t *handover;          // 1 + the step of the handover to the cooperative mode, or 0
-    double *vc, *r, *td, *fei, *ast, *asf, *time, *length;
-};
+// Results: one cor_traj record per trajectory (80 B: status, steps, attempts, handover,
+// then the eight exit doubles), so that a trajectory's end writes five aligned 16-byte
+// stores to neighbouring addresses and the host copies the array to the caller as it is.
+static_assert(sizeof(cor_traj) == 80 && offsetof(cor_traj, vc) == 16, "cor_traj record layout");
 
 // Per-run counters of the tracer: how full the per-lane wavefronts were and when the
 // phases ended (times in wall_clock64 ticks; t_start starts at all ones).
@@ -169,7 +167,7 @@ struct TraceArgs {
     double cos_tu;
     double mass;
     double inv_mass;             // 1 / mass
-    TraceOut out;
+    cor_traj *out;               // result records, indexed by global trajectory id
     // cooperative mode (mixed precision, IGRF/T96 model sets): a trajectory that has
     // run handover_steps accepted steps in the per-lane mode is parked and continued by
     // cooperative workers, a group of lanes per trajectory (coop_on = 0: per-lane only)

@@ -316,19 +316,11 @@ __device__ __forceinline__ int find_scan(const ScanDev *__restrict__ scans, int 
 // nothing of the previous one).
 __device__ __forceinline__ void write_end(const TraceArgs &a, int64_t gid, int status, unsigned int step,
                                           uint32_t attempts, uint32_t handover = 0) {
-    a.out.status[gid] = status;
-    a.out.steps[gid] = step;
-    a.out.attempts[gid] = attempts;
-    a.out.handover[gid] = handover;
+    unsigned char *rec = reinterpret_cast<unsigned char *>(a.out + gid);
+    *reinterpret_cast<uint4 *>(rec) = make_uint4((uint32_t)status, step, attempts, handover);
     if (status != ST_ALLOWED) {
-        a.out.vc[gid] = 0.;
-        a.out.r[gid] = 0.;
-        a.out.td[gid] = 0.;
-        a.out.fei[gid] = 0.;
-        a.out.ast[gid] = 0.;
-        a.out.asf[gid] = 0.;
-        a.out.time[gid] = 0.;
-        a.out.length[gid] = 0.;
+        const double2 z = make_double2(0., 0.);
+        for (int k = 0; k < 4; k++) reinterpret_cast<double2 *>(rec + 16)[k] = z;
     }
 }
 
@@ -385,14 +377,11 @@ __device__ __forceinline__ void write_exit(const TraceArgs &a, int64_t gid, v3<d
     const double dm = sqrt(vth * vth + va * va);
     double asf = (ph + atan2(vph, va)) / K_TO_RAD;
     if (asf > 360.) asf -= 360.;
-    a.out.vc[gid] = v / K_C;
-    a.out.r[gid] = r;
-    a.out.td[gid] = 90. - (th / K_TO_RAD);
-    a.out.fei[gid] = ph / K_TO_RAD;
-    a.out.ast[gid] = atan2(-vth * st + vr * ct, dm) / K_TO_RAD;
-    a.out.asf[gid] = asf;
-    a.out.time[gid] = time;
-    a.out.length[gid] = alength;
+    double2 *d = reinterpret_cast<double2 *>(reinterpret_cast<unsigned char *>(a.out + gid) + 16);
+    d[0] = make_double2(v / K_C, r);
+    d[1] = make_double2(90. - (th / K_TO_RAD), ph / K_TO_RAD);
+    d[2] = make_double2(atan2(-vth * st + vr * ct, dm) / K_TO_RAD, asf);
+    d[3] = make_double2(time, alength);
 }
 
 // src/trajectory.c:46-152 for one lane, FP64 and in the reference's operation order:

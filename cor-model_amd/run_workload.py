@@ -26,6 +26,14 @@ stderr).  All inputs are synthetic and seeded; the field models are the library'
     python cor-model_amd/run_workload.py storm [--epochs 288] [--sites 50]
     python cor-model_amd/run_workload.py map [--lat-step 1] [--fp64-frac 0.01]
     python cor-model_amd/run_workload.py directions [--sites 64]
+    python cor-model_amd/run_workload.py modes
+
+  modes       the benchmark grid (5-degree global grid, IGRF-14 + T96, 0.01-20 GV)
+              split into the tracer's two modes, one launch each, so that a counter
+              run sees each mode apart: the per-lane mode (the grid at step limit
+              65 536, the handover step, cooperative mode off), then the cooperative
+              mode (the trajectories that reach 65 536 steps, each a single-rigidity
+              scan traced cooperatively from its first step to the 1e6 step limit).
 
 Throughput is trajectories/s and accepted steps/s over the tracer kernel's own time
 (HIP events on its stream); the roofline fraction uses the op-count flop model of the
@@ -363,6 +371,45 @@ def run_directions(args):
     return out
 
 
+def run_modes(args):
+    sites = A.grid_sites(5.0, 5.0, 20.0)
+    rows = vertical_rows(sites, 0.01, 0.01, 20.0, args.nk1)
+    ms = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
+    params = A.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2.0, -20.0, 0.0, -2.0])
+    ho = 65536
+    log(f"modes: {len(rows)} sites x 2000 rigidities; per-lane launch at step limit {ho}")
+    pl = A.Plan(ms, params, rows, precision=A.PREC_MIXED, step_limit=ho, queue_order=1, coop=-1)
+    st_l, _ = timed_run(pl, "per-lane mode")
+    tl, _ = pl.results()
+    pl.close()
+    per = 2000
+    hit = np.flatnonzero(tl["steps"] >= ho)
+    scan_of, rig_i = hit // per, hit % per
+    one = [(rows[k]["r0"], rows[k]["the0"], rows[k]["fi0"], rows[k]["the1"], rows[k]["fi1"],
+            rows[k]["rig"] + i * rows[k]["del_"], rows[k]["del_"], rows[k]["rig"] + i * rows[k]["del_"],
+            rows[k]["zn"], args.nk1) for k, i in zip(scan_of, rig_i)]
+    crow = A.scans_array(one)
+    log(f"cooperative launch: {len(crow)} trajectories from their first step, step limit {args.step_limit}")
+    pc = A.Plan(ms, params, crow, precision=A.PREC_MIXED, step_limit=args.step_limit, queue_order=1, coop=1,
+                handover_steps=1)
+    st_c, _ = timed_run(pc, "cooperative mode")
+    tc, _ = pc.results()
+    pc.close()
+
+    def block(st):
+        b = rate_block(st, "t96_igrf")
+        b.update({"wave_iters": int(st.wave_iters), "lane_iters": int(st.lane_iters),
+                  "lane_fill": st.lane_iters / max(64 * st.wave_iters, 1),
+                  "coop_wavefront_steps": int(st.coop_steps), "grid_blocks": int(st.grid_blocks),
+                  "block_threads": int(st.block_threads), "drain_ms": st.drain_ms,
+                  "lane_end_ms": st.lane_end_ms})
+        return b
+    return {"workload": "benchmark grid split by tracer mode (IGRF-14 + T96, mixed)",
+            "per_lane": block(st_l), "cooperative": block(st_c),
+            "reached_handover": int(len(hit)),
+            "cooperative_step_limit_ended": int((tc["steps"] >= args.step_limit).sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -381,7 +428,8 @@ def main():
     d = sub.add_parser("directions")
     d.add_argument("--sites", type=int, default=64)
     d.add_argument("--fp64-scans", type=int, default=96)
-    for p in (s, m, d):
+    o = sub.add_parser("modes")
+    for p in (s, m, d, o):
         p.add_argument("--seed", type=int, default=2020)
         p.add_argument("--nk1", type=int, default=400)
         p.add_argument("--step-limit", type=int, default=1000000)
@@ -389,7 +437,7 @@ def main():
     args = ap.parse_args()
     A.init(0)
     t0 = time.perf_counter()
-    out = {"storm": run_storm, "map": run_map, "directions": run_directions}[args.cmd](args)
+    out = {"storm": run_storm, "map": run_map, "directions": run_directions, "modes": run_modes}[args.cmd](args)
     out["total_wall_s"] = time.perf_counter() - t0
     print(json.dumps(out), flush=True)
 

@@ -226,10 +226,8 @@ int cor_gpu_trace_epochs(const cor_model_set *models, const ModelsParams *epochs
  * the reference.  Nothing is allocated: the epoch block, the scan descriptors and the
  * run's counters live in a fixed scratch array of about 0.5 MB that each GPU holds for
  * the life of the process; the caller passes device buffers for the results:
- *   d_traj  cor_scan_count summed over the scans (n) x COR_TRAJ_SOA_BYTES bytes, 8-byte
- *           aligned: int32 status[n], uint32 steps[n], attempts[n], handover[n], then
- *           double vc[n], r[n], td[n], fei[n], ast[n], asf[n], time[n], length_m[n]
- *           (the fields of cor_traj, one array each);
+ *   d_traj  cor_scan_count summed over the scans cor_traj records, 16-byte aligned
+ *           (COR_E_INVALID otherwise), in scan order, rigidity ascending;
  *   d_cut   n_scans cor_cutoff.
  * At most COR_NOALLOC_MAX_SCANS scans (COR_E_INVALID beyond); no crustal model
  * (COR_E_UNSUPPORTED).  Always the per-lane mode, whose results equal cor_gpu_trace's
@@ -238,9 +236,8 @@ int cor_gpu_trace_epochs(const cor_model_set *models, const ModelsParams *epochs
  * place.  Every call on a GPU uses that GPU's one scratch array: calls that overlap on
  * one GPU race. */
 #define COR_NOALLOC_MAX_SCANS 4096
-#define COR_TRAJ_SOA_BYTES 80
 int cor_gpu_trace_noalloc(const cor_model_set *models, const ModelsParams *epoch, const cor_scan *scans,
-                          int32_t n_scans, const cor_trace_opts *opts, void *d_traj, cor_cutoff *d_cut,
+                          int32_t n_scans, const cor_trace_opts *opts, cor_traj *d_traj, cor_cutoff *d_cut,
                           void *stream);
 
 /* Per-step tracks of every trajectory of the scans in FP64, as the reference's

@@ -12,11 +12,6 @@ import cor_amd as A
 
 pytestmark = pytest.mark.gpu
 
-SOA = [("status", np.int32), ("steps", np.uint32), ("attempts", np.uint32), ("handover", np.uint32),
-       ("vc", np.float64), ("r", np.float64), ("td", np.float64), ("fei", np.float64), ("ast", np.float64),
-       ("asf", np.float64), ("time", np.float64), ("length_m", np.float64)]
-
-
 class Hip:
     def __init__(self):
         self.h = C.CDLL("libamdhip64.so.7")
@@ -42,26 +37,21 @@ def noalloc_trace(hip, ms, p, rows, opts):
                                           C.c_void_p, C.c_void_p]
     lib.cor_gpu_trace_noalloc.restype = C.c_int
     n = sum(int(round((r["rk"] - r["rig"]) / r["del_"])) + 1 for r in rows)
-    d_traj = hip.malloc(n * 80)
+    rec = A.TRAJ_DTYPE.itemsize         # one cor_traj record (80 B) per trajectory
+    d_traj = hip.malloc(n * rec)
     d_cut = hip.malloc(len(rows) * A.CUT_DTYPE.itemsize)
     # fill with garbage so that anything the call fails to write shows
-    assert hip.h.hipMemset(d_traj, 0x5a, n * 80) == 0
+    assert hip.h.hipMemset(d_traj, 0x5a, n * rec) == 0
     try:
         rc = lib.cor_gpu_trace_noalloc(C.addressof(ms), C.addressof(p), rows.ctypes.data, len(rows),
                                        C.addressof(opts), d_traj, d_cut, None)
         assert rc == 0, (rc, lib.cor_gpu_last_hip_error())
-        raw = hip.to_host(d_traj, n * 80)
+        raw = hip.to_host(d_traj, n * rec)
         cut = hip.to_host(d_cut, len(rows) * A.CUT_DTYPE.itemsize).view(A.CUT_DTYPE)
     finally:
         hip.h.hipFree(d_traj)
         hip.h.hipFree(d_cut)
-    tr = np.zeros(n, dtype=A.TRAJ_DTYPE)
-    off = 0
-    for name, dt in SOA:
-        size = np.dtype(dt).itemsize * n
-        tr[name] = raw[off:off + size].view(dt)
-        off += size
-    return tr, cut
+    return raw.view(A.TRAJ_DTYPE), cut
 
 
 @pytest.mark.parametrize("precision", [A.PREC_MIXED, A.PREC_FP64], ids=["mixed", "fp64"])
