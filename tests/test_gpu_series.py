@@ -55,10 +55,10 @@ def test_ts05_series_matches_single_epochs(cor, in_flight):
     params = [cor.recalc(*t, parmod=pm) for t, pm in ep]
     rows = site_rows(cor, SITES)
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_TS05)
-    tr, cut = cor.trace_series(ms, params, rows, in_flight=in_flight, step_limit=20000)
+    tr, cut = cor.trace_series(ms, params, rows, in_flight=in_flight, step_limit=8000)
     assert tr.shape == (4, 3 * 39) and cut.shape == (4, 3)
     for e, p in enumerate(params):
-        t1, c1 = cor.trace(ms, p, rows, step_limit=20000)
+        t1, c1 = cor.trace(ms, p, rows, step_limit=8000)
         assert np.array_equal(tr[e], t1), e
         assert np.array_equal(cut[e], c1), e
     print("series cutoffs (Rc):", cut["rc"].tolist())
@@ -101,13 +101,13 @@ def test_multi_epoch_launch_equals_single_epochs(cor, precision, ext):
     rows = site_rows(cor, SITES)
     ms = cor.model_set(cor.INTERNAL_IGRF, ext)
     for qo in ((1,) if precision == 0 else (0, 1)):
-        tr, cut = cor.trace_epochs(ms, params, rows, step_limit=20000, precision=precision, queue_order=qo)
+        tr, cut = cor.trace_epochs(ms, params, rows, step_limit=8000, precision=precision, queue_order=qo)
         assert tr.shape == (6, 3 * 39) and cut.shape == (6, 3)
         for e, p in enumerate(params):
-            t1, c1 = cor.trace(ms, p, rows, step_limit=20000, precision=precision, coop=-1)
+            t1, c1 = cor.trace(ms, p, rows, step_limit=8000, precision=precision, coop=-1)
             assert np.array_equal(tr[e], t1), (qo, e)
             assert np.array_equal(cut[e], c1), (qo, e)
-    plan = cor.Plan(ms, params, rows, step_limit=20000, precision=precision, queue_order=1)
+    plan = cor.Plan(ms, params, rows, step_limit=8000, precision=precision, queue_order=1)
     plan.run()
     tp, cp = plan.results()
     st = plan.stats()
@@ -176,7 +176,7 @@ def test_set_epoch_cooperative_tables_follow_the_epoch(cor):
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
     quiet = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     storm = cor.recalc(2020, 1, 1, 1, 13, 20, 0, parmod=[8, -250, 4, -12])
-    kw = dict(step_limit=200000, precision=cor.PREC_MIXED, handover_steps=1)
+    kw = dict(step_limit=60000, precision=cor.PREC_MIXED, handover_steps=1)
     plan = cor.Plan(ms, quiet, rows, **kw)
     plan.run()
     plan.results()
@@ -186,7 +186,7 @@ def test_set_epoch_cooperative_tables_follow_the_epoch(cor):
     plan.close()
     tf, cf = cor.trace(ms, storm, rows, **kw)
     assert np.array_equal(tg, tf) and np.array_equal(cg, cf)
-    _, c64 = cor.trace(ms, storm, rows, step_limit=200000)
+    _, c64 = cor.trace(ms, storm, rows, step_limit=60000)
     for a, d in zip(cg, c64):
         tol64 = max(0.25, 0.5 * (d["ru"] - d["rl"]))
         print("cooperative", a["rc"], "fp64", d["rc"], "tol", tol64)

@@ -159,7 +159,7 @@ def test_fp64_cooperative_is_deterministic(cor):
     rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
-    out = [cor.trace(ms, p, rows, step_limit=100000, precision=cor.PREC_FP64, coop=1, handover_steps=300,
+    out = [cor.trace(ms, p, rows, step_limit=30000, precision=cor.PREC_FP64, coop=1, handover_steps=300,
                      queue_order=q) for q in (1, 1, 0)]
     print(len(rows), "scans,", int((out[0][0]["handover"] > 0).sum()), "trajectories cooperative")
     assert int((out[0][0]["handover"] > 0).sum()) > 0
@@ -298,8 +298,8 @@ def test_queue_order_leaves_results_unchanged(cor, precision, coop):
     rows = cor.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sites])
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
-    out = [cor.trace(ms, p, rows, step_limit=100000, precision=precision, coop=coop, queue_order=q)
-           for q in (0, 1)]
+    out = [cor.trace(ms, p, rows, step_limit=100000 if precision else 30000, precision=precision, coop=coop,
+                     queue_order=q) for q in (0, 1)]
     (t0, c0), (t1, c1) = out
     assert np.array_equal(t0, t1) and np.array_equal(c0, c1)
     print(len(rows), "scans,", len(t0), "trajectories identical in both orders")

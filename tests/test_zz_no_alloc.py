@@ -63,9 +63,10 @@ def test_noalloc_equals_cor_gpu_trace(precision):
     sites = A.grid_sites()[::211]
     for k, sub in enumerate((sites, sites[3:5])):
         rows = A.scans_array([(r0, lat, lon, lat, lon, 0.05, 0.05, 20.0, -1.0, 100) for r0, lat, lon in sub])
-        opts = A.TraceOpts(100000, 0.01, A.M_PROTON, precision, 1, -1, 0)
+        limit = 100000 if precision == A.PREC_MIXED else 30000
+        opts = A.TraceOpts(limit, 0.01, A.M_PROTON, precision, 1, -1, 0)
         got_t, got_c = noalloc_trace(hip, ms, p, rows, opts)
-        want_t, want_c = A.trace(ms, p, rows, step_limit=100000, precision=precision, coop=-1)
+        want_t, want_c = A.trace(ms, p, rows, step_limit=limit, precision=precision, coop=-1)
         print("call", k + 1, len(rows), "scans,", len(got_t), "trajectories; cutoffs", got_c["rc"][:3])
         assert np.array_equal(got_t, want_t), int((got_t != want_t).sum())
         assert np.array_equal(got_c, want_c)
