@@ -306,17 +306,21 @@ NoAllocHost g_noalloc_host[NOALLOC_MAX_DEVICES];
 // 5-degree IGRF + T96 grid the median trajectory runs 435 steps, 0.8 % run past 65536 and
 // hold 40 % of all steps; about 1 in 3000 reaches the 10^6-step limit.  A per-lane step
 // costs about a quarter of a cooperative one per trajectory, but a cooperative step has
-// a quarter of its latency, which the trapped trajectories need.  Full grid, measured:
-// 16.8 s (49152), 16.2 s (65536), 15.8 s (98304), 17.8 s (131072), 17.9 s (16384); 65536
-// is taken because a single scan, whose run time is the latency of its longest
-// trajectory, then leaves the per-lane mode sooner.
-constexpr uint32_t COOP_HANDOVER_DEFAULT = 65536u;
+// a quarter of its latency, which the trapped trajectories need.  Mixed precision, with
+// the per-lane compaction keeping the drain's wavefronts full, full grid on one host:
+// 15.5-15.8 s (65536), 15.3 s (81920), 15.4 s (90112), 14.8-15.0 s (98304), 15.0 s
+// (106496), 15.3 s (114688), 15.5 s (131072); the 64-site x 193-direction workload
+// 56.1 s + 61.1 s (98304) against 60.0 s + 66.8 s (65536).  FP64 (its cooperative tail
+// is opt-in): 68.2 s (65536) against 68.8 s (98304).
+constexpr uint32_t COOP_HANDOVER_DEFAULT_MIXED = 98304u;
+constexpr uint32_t COOP_HANDOVER_DEFAULT_FP64 = 65536u;
 // An earlier handover for the lowest rigidities of each scan, which hold most of the
 // trajectories trapped until the step limit (on the 5-degree grid 1468 of 1852 lie among
 // the first 20 of 2000), is off by default: it moves more work into the cooperative mode
-// than it saves on the trapped trajectories' path.  Full grid, measured against 15.9 s
-// with every trajectory at 65536: 16.8 s (32768 for the lowest 20), 16.8 s (16384, 20),
-// 16.6 s (32768, 50), 16.8 s (98304 for the others, 32768 for the lowest 20).
+// than it saves on the trapped trajectories' path.  Full grid, measured (before the
+// per-lane compaction) against 15.9 s with every trajectory at 65536: 16.8 s (32768 for
+// the lowest 20), 16.8 s (16384, 20), 16.6 s (32768, 50), 16.8 s (98304 for the others,
+// 32768 for the lowest 20).
 constexpr uint32_t COOP_HANDOVER_LOW_DEFAULT = 32768u;
 constexpr int32_t COOP_HANDOVER_LOW_COUNT_DEFAULT = 0;
 // Parked trajectories waiting for workers, beyond eight per converting wavefront, that
@@ -599,7 +603,9 @@ int plan_create_impl(const cor_model_set *models, const ModelsParams *epochs, in
     // so does the multi-epoch tracer)
     p->coop_on = !multi && (p->prec == COR_PREC_MIXED ? opts->coop >= 0 : opts->coop > 0) && ext != EXT_TS05 &&
                  cru == CRU_NONE;
-    p->handover_steps = opts->handover_steps > 0 ? (uint32_t)opts->handover_steps : COOP_HANDOVER_DEFAULT;
+    p->handover_steps = opts->handover_steps > 0 ? (uint32_t)opts->handover_steps
+                        : p->prec == COR_PREC_FP64 ? COOP_HANDOVER_DEFAULT_FP64
+                                                   : COOP_HANDOVER_DEFAULT_MIXED;
     // the lowest rigidities hand over no later than the others
     p->handover_low_steps = opts->handover_low_steps > 0   ? (uint32_t)opts->handover_low_steps
                             : opts->handover_low_steps < 0 ? p->handover_steps

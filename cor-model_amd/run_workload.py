@@ -31,9 +31,10 @@ stderr).  All inputs are synthetic and seeded; the field models are the library'
   modes       the benchmark grid (5-degree global grid, IGRF-14 + T96, 0.01-20 GV)
               split into the tracer's two modes, one launch each, so that a counter
               run sees each mode apart: the per-lane mode (the grid at step limit
-              65 536, the handover step, cooperative mode off), then the cooperative
-              mode (the trajectories that reach 65 536 steps, each a single-rigidity
-              scan traced cooperatively from its first step to the 1e6 step limit).
+              98 304, the mixed tracer's handover step, cooperative mode off), then the
+              cooperative mode (the trajectories that reach 98 304 steps, each a
+              single-rigidity scan traced cooperatively from its first step to the 1e6
+              step limit).
 
 Throughput is trajectories/s and accepted steps/s over the tracer kernel's own time
 (HIP events on its stream); the roofline fraction uses the op-count flop model of the
@@ -331,18 +332,21 @@ def run_directions(args):
     ms = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
     params = A.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2.0, -20.0, 0.0, -2.0])
     log(f"directions: {args.sites} sites x {len(dirs)} directions x 2000 rigidities")
-    pm = A.Plan(ms, params, rows, precision=A.PREC_MIXED, step_limit=args.step_limit, queue_order=1)
+    pm = A.Plan(ms, params, rows, precision=A.PREC_MIXED, step_limit=args.step_limit, queue_order=1,
+                handover_steps=args.handover_steps)
     st_c, wall_c = timed_run(pm, "mixed coarse")
     tm, cm = pm.results()
     pm.close()
     fine = A.refine_scans(rows, cm, 10)
-    pf = A.Plan(ms, params, fine, precision=A.PREC_MIXED, step_limit=args.step_limit, queue_order=1)
+    pf = A.Plan(ms, params, fine, precision=A.PREC_MIXED, step_limit=args.step_limit, queue_order=1,
+                handover_steps=args.handover_steps)
     st_f, wall_f = timed_run(pf, "mixed refinement (0.001 GV over each penumbra band)")
     tf, cf = pf.results()
     pf.close()
     allowed = tm["status"] == 0
     out = {"workload": "apparent cutoffs and asymptotic directions, IGRF-14 + T96, refined penumbra",
            "sites": args.sites, "directions": len(dirs), "scans": len(rows), "nk1": args.nk1,
+           "handover_steps": args.handover_steps,
            "mixed_coarse": rate_block(st_c, "t96_igrf"), "mixed_refine": rate_block(st_f, "t96_igrf"),
            "refined_trajectories": int(st_f.n_traj), "allowed_with_asymptotic_direction": int(allowed.sum()),
            "wall_s": wall_c + wall_f,
@@ -376,7 +380,7 @@ def run_modes(args):
     rows = vertical_rows(sites, 0.01, 0.01, 20.0, args.nk1)
     ms = A.model_set(A.INTERNAL_IGRF, A.EXTERNAL_T96)
     params = A.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2.0, -20.0, 0.0, -2.0])
-    ho = 65536
+    ho = 98304
     log(f"modes: {len(rows)} sites x 2000 rigidities; per-lane launch at step limit {ho}")
     pl = A.Plan(ms, params, rows, precision=A.PREC_MIXED, step_limit=ho, queue_order=1, coop=-1)
     st_l, _ = timed_run(pl, "per-lane mode")
@@ -428,6 +432,8 @@ def main():
     d = sub.add_parser("directions")
     d.add_argument("--sites", type=int, default=64)
     d.add_argument("--fp64-scans", type=int, default=96)
+    d.add_argument("--handover-steps", type=int, default=0,
+                   help="mixed tracer: per-lane steps before the cooperative mode (0: library default)")
     o = sub.add_parser("modes")
     for p in (s, m, d, o):
         p.add_argument("--seed", type=int, default=2020)

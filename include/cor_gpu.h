@@ -87,8 +87,8 @@ typedef struct {
     int32_t handover_steps;   /* with the cooperative mode: accepted steps a trajectory runs in
                                * the per-lane mode (one lane each) before it continues in the
                                * cooperative mode (a group of lanes each); 0 = the library
-                               * default (65536).  A fixed step count: the results do not
-                               * depend on scheduling */
+                               * default (98304 in mixed precision, 65536 in FP64).  A fixed
+                               * step count: the results do not depend on scheduling */
     int32_t waves_per_simd;   /* resident wavefronts per SIMD of the persistent grid: 0 = as many
                                * as the kernel's registers allow (the default), otherwise at most
                                * that many.  Scheduling only: the results do not change */
