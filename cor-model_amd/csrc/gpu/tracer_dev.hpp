@@ -870,8 +870,14 @@ __device__ bool coop_worker(const TraceArgs &a, uint32_t run_id, const EpochView
 // for them).  A pooled trajectory continues in the per-lane mode, with the same
 // arithmetic, so compaction changes no result.  A wavefront that took from the pool
 // runs POOL_HOLD_ITERS steps before it may give to it again.
-constexpr int POOL_DONATE_LANES = 16;
-constexpr unsigned POOL_HOLD_ITERS = 256;
+#ifndef COR_POOL_DONATE_LANES
+#define COR_POOL_DONATE_LANES 16
+#endif
+#ifndef COR_POOL_HOLD_ITERS
+#define COR_POOL_HOLD_ITERS 256
+#endif
+constexpr int POOL_DONATE_LANES = COR_POOL_DONATE_LANES;
+constexpr unsigned POOL_HOLD_ITERS = COR_POOL_HOLD_ITERS;
 
 __device__ __forceinline__ void pool_write(ParkDev *P, uint32_t run_id, int64_t gid, v3<double> pos, v3<double> vel,
                                            double v, double time, double alength, double zn, unsigned int step,

@@ -18,6 +18,7 @@
 #include "devmath.hpp"
 #include "t96_dev.hpp"
 #include "ts05_tables.h"
+#include "fastmath64.hpp"
 
 namespace cor {
 
@@ -33,6 +34,8 @@ struct TS05Params {
     T wrp[2][10];                // their 1/P, 1/R scales
     T wsq[2][25];                // sqrt(1/P^2 + 1/R^2)
     T r11[9], r21[9], src[9], prc[9];   // shield coefficient sums a0 + s a1 + a2 + s a3
+    T shinv[4][6];               // the four shields' 1/P_j, 1/R_k (order r11, r21, src, prc)
+    T shsq[4][9];                // and their sqrt(1/P_j^2 + 1/R_k^2)
 };
 
 // Forward-mode dual number: value and derivative along one direction.
@@ -218,22 +221,37 @@ __device__ __forceinline__ v3<T> taildisk(T d0, T deltadx, T x, T y, T z) {
     return mk3<T>(dbx, dby, dbz);
 }
 
+// e^x and sin/cos of the FP64 harmonic sums: in the mixed path (FAST) one shared
+// reduction and Taylor series (fastmath64.hpp), within a few ulp; the FP64 path keeps
+// the library's correctly rounded routines.
+template <class T, bool FAST>
+__device__ __forceinline__ T hexp(T v) {
+    if constexpr (FAST) return fm64::exp(v);
+    else return Dm<T>::exp(v);
+}
+template <class T, bool FAST>
+__device__ __forceinline__ void hsincos(T v, T *s, T *c) {
+    if constexpr (FAST) fm64::sincos(v, s, c);
+    else Dm<T>::sincos(v, s, c);
+}
+
 // lib/ts05/deformed.c:182-208 with the per-epoch coefficients and scales
-template <class T>
+template <class T, bool FAST = false>
 __device__ __forceinline__ v3<T> shlcar5x5(const T *coef, const T *rpr, const T *sqt, T x, T y, T z) {
+    static_assert(!FAST || Dm<T>::fp64, "the fast harmonic-sum functions are FP64");
     T hx = 0, hy = 0, hz = 0;
     T szk[5], czk[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) Dm<T>::sincos(z * rpr[5 + k], &szk[k], &czk[k]);
+    for (int k = 0; k < 5; k++) hsincos<T, FAST>(z * rpr[5 + k], &szk[k], &czk[k]);
     for (int i = 0; i < 5; i++) {
         const T rp = rpr[i];
         T sy, cy;
-        Dm<T>::sincos(y * rp, &sy, &cy);
+        hsincos<T, FAST>(y * rp, &sy, &cy);
 #pragma unroll
         for (int k = 0; k < 5; k++) {
             const T rr = rpr[5 + k];
             const T sq = sqt[5 * i + k];
-            const T e = Dm<T>::exp(x * sq);
+            const T e = hexp<T, FAST>(x * sq);
             const T c = coef[5 * i + k];
             hx = hx + c * (-sq * e * cy * szk[k]);
             hy = hy + c * (rp * e * sy * szk[k]);
@@ -268,7 +286,8 @@ __device__ __forceinline__ void warped(const TS05Params<T> &P, const TS05Params<
         v3<T> f = taildisk(P.d0 * T(1.1), T(1.), (x - T(6.) - P.dxshift1) * T(1.1) + T(1.2), yu * T(1.1), zu * T(1.1));
         pin(f);
         COR_SEQ();
-        const v3<T> h = to3<T>(shlcar5x5<double>(Pd.w1, Pd.wrp[0], Pd.wsq[0], (double)x, (double)yu, (double)zu));
+        const v3<T> h = to3<T>(shlcar5x5<double, !Dm<T>::fp64>(Pd.w1, Pd.wrp[0], Pd.wsq[0], (double)x, (double)yu,
+                                                               (double)zu));
         u[0] = mk3<T>(f.x + h.x, f.y + h.y, f.z + h.z);
         pin(u[0]);
     }
@@ -277,7 +296,8 @@ __device__ __forceinline__ void warped(const TS05Params<T> &P, const TS05Params<
         v3<T> f = taildisk(P.d0 * T(0.25), T(0.), (x - T(4.) - P.dxshift2) * T(0.25) - T(9.), yu * T(0.25), zu * T(0.25));
         pin(f);
         COR_SEQ();
-        const v3<T> h = to3<T>(shlcar5x5<double>(Pd.w2, Pd.wrp[1], Pd.wsq[1], (double)x, (double)yu, (double)zu));
+        const v3<T> h = to3<T>(shlcar5x5<double, !Dm<T>::fp64>(Pd.w2, Pd.wrp[1], Pd.wsq[1], (double)x, (double)yu,
+                                                               (double)zu));
         u[1] = mk3<T>(f.x + h.x, f.y + h.y, f.z + h.z);
         pin(u[1]);
     }
@@ -438,25 +458,40 @@ __device__ __forceinline__ v3<T> birk_1n2(const TS05Params<T> &P, int numb, T x,
 
 // Shield harmonics (lib/ts05/birk_tot.c:379-403, lib/ts05/full_rc.c:697-721) with the
 // per-epoch coefficient sums; pr = P[3], R[3]; fac multiplies each term when use_fac.
-template <class T>
-__device__ __forceinline__ v3<T> shield(const T *coef, const T *pr, bool use_fac, T fac, T x, T y, T z) {
+// FAST (the mixed path): the scales' reciprocals and roots from the per-epoch tables
+// (inv = 1/P, 1/R; sqt = sqrt(1/P^2 + 1/R^2)) and the fast exp and sin/cos.
+template <class T, bool FAST = false>
+__device__ __forceinline__ v3<T> shield(const T *coef, const T *pr, const T *inv, const T *sqt, bool use_fac, T fac,
+                                        T x, T y, T z) {
+    static_assert(!FAST || Dm<T>::fp64, "the fast harmonic-sum functions are FP64");
     T gx = 0, gy = 0, gz = 0;
     T sz[3], cz[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) Dm<T>::sincos(z / pr[3 + k], &sz[k], &cz[k]);
+    for (int k = 0; k < 3; k++) hsincos<T, FAST>(FAST ? z * inv[3 + k] : z / pr[3 + k], &sz[k], &cz[k]);
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const T p = pr[j];
         T sy, cy;
-        Dm<T>::sincos(y / p, &sy, &cy);
-        const T pinv2 = T(1.) / (p * p);
+        hsincos<T, FAST>(FAST ? y * inv[j] : y / p, &sy, &cy);
+        const T pinv2 = FAST ? T(0.) : T(1.) / (p * p);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const T r = pr[3 + k];
-            const T sq = Dm<T>::sqrt(pinv2 + T(1.) / (r * r));
-            const T e = Dm<T>::exp(x * sq);
+            const T sq = FAST ? sqt[3 * j + k] : Dm<T>::sqrt(pinv2 + T(1.) / (r * r));
+            const T e = hexp<T, FAST>(x * sq);
             const T c = coef[3 * j + k];
-            if (use_fac) {
+            if constexpr (FAST) {
+                const T ip = inv[j], ir = inv[3 + k];
+                if (use_fac) {
+                    gx += (-sq * e * cy * sz[k] * fac) * c;
+                    gy += (e * sy * sz[k] * ip * fac) * c;
+                    gz += (-e * cy * cz[k] * ir * fac) * c;
+                } else {
+                    gx += (-sq * e * cy * sz[k]) * c;
+                    gy += (e * sy * sz[k] * ip) * c;
+                    gz += (-e * cy * cz[k] * ir) * c;
+                }
+            } else if (use_fac) {
                 gx += (-sq * e * cy * sz[k] * fac) * c;
                 gy += (e * sy * sz[k] / p * fac) * c;
                 gz += (-e * cy * cz[k] / r * fac) * c;
@@ -807,10 +842,12 @@ __device__ __forceinline__ void full_rc(const TS05Params<T> &P, const TS05Params
     COR_SEQ();
     const T bxq = hq.x * cp + hq.y * sp;
     const T byq = -hq.x * sp + hq.y * cp;
-    v3<T> fs = to3<T>(shield<double>(Pd.src, CT2(k_ts05_csy) + 72, true, Pd.fac_sy, (double)x, (double)y, (double)z));
+    v3<T> fs = to3<T>(shield<double, !Dm<T>::fp64>(Pd.src, CT2(k_ts05_csy) + 72, Pd.shinv[2], Pd.shsq[2], true,
+                                                     Pd.fac_sy, (double)x, (double)y, (double)z));
     pin(fs);
     COR_SEQ();
-    const v3<T> fp = to3<T>(shield<double>(Pd.prc, CT2(k_ts05_cpr) + 72, true, Pd.fac_pr, (double)x, (double)y, (double)z));
+    const v3<T> fp = to3<T>(shield<double, !Dm<T>::fp64>(Pd.prc, CT2(k_ts05_cpr) + 72, Pd.shinv[3], Pd.shsq[3], true,
+                                                           Pd.fac_pr, (double)x, (double)y, (double)z));
     *src = mk3<T>(hs.x + fs.x, hs.y + fs.y, hs.z + fs.z);
     *prc = mk3<T>((ha.x + bxq) + fp.x, (ha.y + byq) + fp.y, (ha.z + hq.z) + fp.z);
 }
@@ -859,7 +896,8 @@ __device__ __forceinline__ v3<T> ts05_field(const TS05Params<T> &P, const TS05Pa
         v3<T> f = birk_1n2(P, 0, xx, yy, zz);
         pin(f);
         COR_SEQ();
-        const v3<T> h = to3<T>(shield<double>(Pd.r11, CT2(k_ts05_sh11) + 72, false, 1., (double)xx, (double)yy, (double)zz));
+        const v3<T> h = to3<T>(shield<double, !Dm<T>::fp64>(Pd.r11, CT2(k_ts05_sh11) + 72, Pd.shinv[0], Pd.shsq[0],
+                                                            false, 1., (double)xx, (double)yy, (double)zz));
         r11 = mk3<T>(f.x + h.x, f.y + h.y, f.z + h.z);
         pin(r11);
     }
@@ -868,7 +906,8 @@ __device__ __forceinline__ v3<T> ts05_field(const TS05Params<T> &P, const TS05Pa
         v3<T> f = birk_1n2(P, 1, xx, yy, zz);
         pin(f);
         COR_SEQ();
-        const v3<T> h = to3<T>(shield<double>(Pd.r21, CT2(k_ts05_sh21) + 72, false, 1., (double)xx, (double)yy, (double)zz));
+        const v3<T> h = to3<T>(shield<double, !Dm<T>::fp64>(Pd.r21, CT2(k_ts05_sh21) + 72, Pd.shinv[1], Pd.shsq[1],
+                                                            false, 1., (double)xx, (double)yy, (double)zz));
         r21 = mk3<T>(f.x + h.x, f.y + h.y, f.z + h.z);
         pin(r21);
     }

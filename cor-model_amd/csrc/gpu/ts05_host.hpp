@@ -3,8 +3,8 @@
 // lib/ts05/ts05.c:19-110 (drivers and amplitudes), lib/ts05/deformed.c:182-208 (warped
 // tail coefficients A[2l] + A[2l+1] * shift and the 1/P, 1/R scales),
 // lib/ts05/birk_tot.c:379-403 and lib/ts05/full_rc.c:697-721 (shield coefficient sums
-// a0 + s a1 + a2 + s a3 and (s + 1)^3), lib/ts05/birk_tot.c:322-328 (tan of the cone
-// band edges).
+// a0 + s a1 + a2 + s a3 and (s + 1)^3, and the shields' scales), lib/ts05/birk_tot.c:322-328
+// (tan of the cone band edges).
 #pragma once
 #include <math.h>
 #include "ts05_dev.hpp"
@@ -76,6 +76,19 @@ inline void ts05_build_params(const double *pm, TS05Params<double> &P) {
         for (int l = 0; l < 9; l++) {
             const double *a = st[t] + 4 * l;
             sc[t][l] = a[0] + xsc[t] * a[1] + a[2] + xsc[t] * a[3];
+        }
+    // the shields' scales for the mixed path's harmonic sums: 1/P_j, 1/R_k and
+    // sqrt(1/P_j^2 + 1/R_k^2) (the last as the FP64 path forms it per term)
+    for (int t = 0; t < 4; t++)
+        for (int j = 0; j < 3; j++) {
+            const double p = st[t][72 + j];
+            P.shinv[t][j] = 1. / p;
+            P.shinv[t][3 + j] = 1. / st[t][75 + j];
+            const double pinv2 = 1. / (p * p);
+            for (int k = 0; k < 3; k++) {
+                const double r = st[t][75 + k];
+                P.shsq[t][3 * j + k] = sqrt(pinv2 + 1. / (r * r));
+            }
         }
     const double x1 = xsc[2] + 1., x2 = xsc[3] + 1.;
     P.fac_sy = x1 * x1 * x1;
