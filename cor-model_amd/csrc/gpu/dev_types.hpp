@@ -97,11 +97,15 @@ struct ParkDev {
 
 // Lanes per trajectory when a cooperative worker runs several trajectories: groups of 8
 // (eight to a wavefront); a lone trajectory gets all 64.  The evaluator also runs with
-// groups of 4 (sixteen to a wavefront; cor_gpu_field's COR_PREC_MIXED_GROUP4), but the
-// tracer is slower with them: on the 5-degree grid 17.7 s against 16.1 s, the trapped
-// trajectories' steps taking longer while the workers are crowded.
+// groups of 4 and 16 (cor_gpu_field's COR_PREC_MIXED_GROUP4 / _GROUP16), and every width
+// gives the same bits, but the tracer is slower with them as the bookkeeping width: on
+// the 5-degree grid 16.3 s (4) and 16.6 s (16) against 14.9 s (8), with the handover at
+// 98304 steps.
+#ifndef COR_COOP_GROUP_MIXED
+#define COR_COOP_GROUP_MIXED 8
+#endif
 constexpr int COOP_GROUP = 8;
-constexpr int COOP_GROUP_MIXED = 8;
+constexpr int COOP_GROUP_MIXED = COR_COOP_GROUP_MIXED;
 template <int PREC>
 constexpr int coop_group() { return PREC == PREC_MIXED ? COOP_GROUP_MIXED : COOP_GROUP; }
 

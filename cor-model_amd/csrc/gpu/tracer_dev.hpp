@@ -635,8 +635,8 @@ __device__ __forceinline__ v3<FT> coop_field_spread(const EpochView<FT> &E, cons
 
 // Cooperative worker; returns true when the run is over, false when the wavefront should
 // go back to the per-lane mode (it is idle, no parked trajectory waits and the queue still
-// has new ones).  A wavefront runs up to 64 / GS parked trajectories (sixteen in mixed
-// precision, eight in FP64), one per group of GS = coop_group<PREC>() lanes, while parked
+// has new ones).  A wavefront runs up to 64 / GS parked trajectories (eight in either
+// precision), one per group of GS = coop_group<PREC>() lanes, while parked
 // work is plentiful; with eight or fewer the field is evaluated with wider groups (a lone
 // trajectory with all 64 lanes: its step then takes about two thirds of the time).  Groups of a wavefront hold the same state in all
 // their lanes and compute the same bits.
