@@ -334,6 +334,13 @@ def test_grid_sites_at_benchmark_resolution_against_reference(cor, precision, co
     p = cor.recalc(2020, 1, 1, 1, 0, 0, 0, parmod=[2, -20, 0, -2])
     ms = cor.model_set(cor.INTERNAL_IGRF, cor.EXTERNAL_T96)
     tr, cut = cor.trace(ms, p, cor.scans_array(rows), step_limit=1000000, precision=precision, coop=coop)
+    # the cooperative tail takes over at the library's default handover step (98 304
+    # accepted steps in mixed precision, 65 536 in FP64), recorded as 1 + that step
+    ho = tr["handover"][tr["handover"] > 0]
+    if precision == 1 or coop > 0:
+        assert len(ho) > 0 and np.all(ho == (98304 if precision == 1 else 65536) + 1), np.unique(ho)
+    else:
+        assert len(ho) == 0
     per = 2000
     within = 0
     for k, (name, (allowed, rcut)) in enumerate(zip(GRID5, refs)):
