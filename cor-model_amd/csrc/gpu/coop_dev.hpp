@@ -885,7 +885,11 @@ __device__ __forceinline__ IgrfShare<F> coop_igrf_share(const F *__restrict__ G,
     for (int pass = 0; pass < NPI; pass++) {
         const int m = r0 + 1 + GL * pass;
         const int m0 = pass == 0 ? 1 : m - GL;    // order the sectoral terms are at
-        for (int j = 1; j < kw; j++) {
+        // the pass's lanes hold orders GL*pass+1 .. GL*(pass+1), so only j in
+        // [GL*(pass-1)+1, GL*(pass+1)) can advance one of them
+        const int j_lo = pass == 0 ? 1 : GL * (pass - 1) + 1;
+        const int j_hi = kw < GL * (pass + 1) ? kw : GL * (pass + 1);
+        for (int j = j_lo; j < j_hi; j++) {
             if (j >= m0 && j < m) {
                 const F w = cm;
                 cm = w * cf + sm * sf;
